@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Throughput of the per-iteration squaring mod 2^p - 1 on one MI355X.
+
+Workload (the reference's single-test throughput configuration): LL test of M_p with
+p = smallest prime >= 70,000,000 + (seed mod 10^5) (p = 70,000,027 for seed 0), NTT length
+2^22 u64 words as the reference's length selection assigns (~16.7 bits/word), start
+residue 4.  A step is one LL iteration (one squaring and -2) over the whole residue, which
+stays resident in HBM.  Prints one JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--seed S] [--cpu-seconds T]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ms/iter @2^22-word NTT (+% HBM roofline)"
+HBM_PEAK_GBS = 8000.0
+
+
+def is_prime(x: int) -> bool:
+    if x < 2:
+        return False
+    if x % 2 == 0:
+        return x == 2
+    i = 3
+    while i * i <= x:
+        if x % i == 0:
+            return False
+        i += 2
+    return True
+
+
+def next_prime(x: int) -> int:
+    while not is_prime(x):
+        x += 1
+    return x
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(p: int, n: int, seconds: float):
+    """The oracle's C restatement of the same weighted NTT squaring (OpenMP over rows and
+    columns), timed on this host for a bounded sample of iterations."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import OracleTest, oracle  # test infrastructure: the CPU baseline leg
+    L = oracle()
+    o = OracleTest(p, n, False, n0=1 << (int(math.log2(n)) // 2))
+    t0 = time.perf_counter()
+    o.iterate(2)
+    per = (time.perf_counter() - t0) / 2
+    k = max(2, int(seconds / max(per, 1e-6)))
+    t0 = time.perf_counter()
+    o.iterate(k)
+    dt = time.perf_counter() - t0
+    cpu = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": round(dt * 1000.0 / k, 3), "unit": "ms/iter", "cores": int(L.lo_threads()),
+            "kind": "port",
+            "sample": f"{k} LL iterations of M{p} at n=2^{int(math.log2(n))} after 2 warm-up "
+                      f"iterations ({dt:.1f} s), oracle/lnt_oracle.c, CPU: {cpu}"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--profile-iters", type=int, default=50)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the hot path is a single test on one device "
+              f"(independent tests shard across GPUs without communication); --gpus {args.gpus} "
+              f"is not run.", file=sys.stderr)
+        return 2
+
+    import cuda_amd
+    from cuda_amd import LucasPRP
+
+    p = next_prime(70_000_000 + args.seed % 100_000)
+    t = LucasPRP(p)
+    n = t.n
+    log(f"bench: p={p} n={n} plan={t.describe()}")
+
+    t.iterate(args.warmup)
+    t.sync()
+    t0 = time.perf_counter()
+    t.sync()
+    t.iterate(args.steps)
+    gpu_ms = t.sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1000.0 / args.steps
+    gpu_ms_per_step = gpu_ms / args.steps
+    done = args.warmup + args.steps
+
+    # parity spot check against the GMP fixture, when this iteration count has one
+    check = None
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from helpers import golden
+        g = golden(p, "LL") or {}
+        if done in g:
+            check = t.res64() == g[done]
+    except Exception as e:  # fixtures are optional for the measurement
+        log("bench: no fixture check:", e)
+
+    prof = t.profile(args.profile_iters)
+    name, kms = max(prof, key=lambda kv: kv[1])
+    alg_bytes = 16 * n  # one read and one write of the n-word residue per launch
+    achieved = alg_bytes / (kms * 1e-3) / 1e9
+    iter_ms = sum(v for _, v in prof)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_2p22.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get(name.split("<")[0])
+        except Exception:
+            traffic = None
+    line = {
+        "metric": METRIC,
+        "value": round(ms_per_step, 5),
+        "unit": "ms/iter",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": f"LL test of M{p}, n=2^22 u64 words (GF(2^64-2^32+1) IBDWT), "
+                               f"iterations {args.warmup + 1}..{done}",
+                   "p": p, "n": n, "plan": t.describe()},
+        "gpu_ms_per_step": round(gpu_ms_per_step, 5),
+        "res64": f"{t.res64():016X}",
+        "res64_matches_gmp": check,
+        "kernels_ms": {k: round(v, 5) for k, v in prof},
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "iteration": {"bytes": 32 * n, "ms": round(iter_ms, 5),
+                                   "achieved": round(32 * n / (iter_ms * 1e-3) / 1e9, 1),
+                                   "frac": round(32 * n / (iter_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(p, n, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

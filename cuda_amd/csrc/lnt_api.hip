@@ -1,0 +1,490 @@
+// C ABI of the engine (include/lnt_api.h): context lifetime, tables, the iteration
+// launcher, readout, timing and the validating tuner.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/lnt_api.h"
+#include "lnt_launch.hpp"
+
+using namespace lnt;
+
+// ------------------------------------------------------------------ table kernels
+
+__global__ void lnt_k_weights(u64 Bw, u64 Binv, u64 ninv, u32 p, u32 n, u64 *w, u64 *uw) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i == 0) {
+        w[0] = 1;
+        uw[0] = ninv;
+        return;
+    }
+    u32 r = (u32)((u64)p * i) & (n - 1);
+    w[i] = gl_pow(Bw, (u64)(n - r));
+    uw[i] = gl_mul(gl_pow(Binv, (u64)(n - r)), ninv);
+}
+
+__global__ void lnt_k_line_tw(u64 om, u32 B1, u32 B2, u32 L, u64 *tw) {
+    u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L) return;
+    tw[i] = lnt_line_tw(om, B1, B2, i);
+}
+
+__global__ void lnt_k_tw4(u64 om, u32 C, u32 B1, u32 B2, u32 n, u64 *tw) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    tw[i] = lnt_tw4(om, C, B1, B2, i);
+}
+
+// ------------------------------------------------------------------ context
+
+struct lnt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_mark = nullptr;
+    Plan pl;
+    uint32_t p = 0;
+    bool prp = false;
+    bool pending = false;     // x holds pre-carry values whose LL "-2" is still due
+    bool pristine = true;     // no iteration since the residue was set
+    bool zero_ones = false;   // residue is 0 mod M_p but was given as a nonzero vector
+    std::vector<uint64_t> set_digits;  // digits as given (returned while pristine)
+    std::vector<uint8_t> wid;
+    u64 *x = nullptr, *y = nullptr, *w = nullptr, *uw = nullptr;
+    u64 *tw4 = nullptr, *itw4 = nullptr, *twr = nullptr, *itwr = nullptr, *twc = nullptr, *itwc = nullptr;
+    LntParams P{};
+    std::string err;
+};
+
+static std::mutex g_err_mu;
+static std::string g_err;
+
+static int fail(lnt_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    else {
+        std::lock_guard<std::mutex> g(g_err_mu);
+        g_err = msg;
+    }
+    return code;
+}
+
+#define HIPCHK(c, call, what)                                                                  \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail((c), e_ == hipErrorOutOfMemory ? LNT_OUT_OF_MEMORY : LNT_DEVICE_ERROR, \
+                        std::string(what) + " (" + hipGetErrorString(e_) + " at " __FILE__    \
+                        " line " + std::to_string(__LINE__) + ")");                            \
+    } while (0)
+
+static void free_buffers(lnt_ctx *c) {
+    u64 **bufs[] = {&c->x, &c->y, &c->w, &c->uw, &c->tw4, &c->itw4, &c->twr, &c->itwr, &c->twc, &c->itwc};
+    for (u64 **b : bufs)
+        if (*b) {
+            (void)hipFree(*b);
+            *b = nullptr;
+        }
+}
+
+template <class T>
+static GView<T> view(T *p, u64 n) {
+    return GView<T>{p, n};
+}
+
+// Allocate and fill every table of plan c->pl on c->stream.
+static int build_tables(lnt_ctx *c) {
+    const Plan &pl = c->pl;
+    const u32 n = pl.n;
+    const size_t nb = (size_t)n * 8;
+    HIPCHK(c, hipMalloc(&c->x, nb), "hipMalloc (x)");
+    HIPCHK(c, hipMalloc(&c->w, nb), "hipMalloc (weights)");
+    HIPCHK(c, hipMalloc(&c->uw, nb), "hipMalloc (unweights)");
+    const u32 Lr = pl.small ? n : pl.C;
+    HIPCHK(c, hipMalloc(&c->twr, (size_t)Lr * 8), "hipMalloc (row twiddles)");
+    HIPCHK(c, hipMalloc(&c->itwr, (size_t)Lr * 8), "hipMalloc (row twiddles)");
+    if (!pl.small) {
+        HIPCHK(c, hipMalloc(&c->y, nb), "hipMalloc (work buffer)");
+        HIPCHK(c, hipMalloc(&c->tw4, nb), "hipMalloc (4-step twiddles)");
+        HIPCHK(c, hipMalloc(&c->itw4, nb), "hipMalloc (4-step twiddles)");
+        HIPCHK(c, hipMalloc(&c->twc, (size_t)pl.R * 8), "hipMalloc (column twiddles)");
+        HIPCHK(c, hipMalloc(&c->itwc, (size_t)pl.R * 8), "hipMalloc (column twiddles)");
+    }
+    const u64 Bw = gl_pow(7, (GL_P - 1) / 192 / n * 5);
+    const u64 ninv = gl_inv(n);
+    hipLaunchKernelGGL(lnt_k_weights, dim3((n + 255) / 256), dim3(256), 0, c->stream, Bw, gl_inv(Bw), ninv, c->p,
+                       n, c->w, c->uw);
+    u64 omr = gl_pow(554, (GL_P - 1) / Lr);
+    hipLaunchKernelGGL(lnt_k_line_tw, dim3((Lr + 255) / 256), dim3(256), 0, c->stream, omr, pl.row.B1, pl.row.B2, Lr,
+                       c->twr);
+    hipLaunchKernelGGL(lnt_k_line_tw, dim3((Lr + 255) / 256), dim3(256), 0, c->stream, gl_inv(omr), pl.row.B1,
+                       pl.row.B2, Lr, c->itwr);
+    if (!pl.small) {
+        u64 omc = gl_pow(554, (GL_P - 1) / pl.R);
+        hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, c->stream, omc, pl.col.B1,
+                           pl.col.B2, pl.R, c->twc);
+        hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, c->stream, gl_inv(omc), pl.col.B1,
+                           pl.col.B2, pl.R, c->itwc);
+        u64 omn = gl_pow(554, (GL_P - 1) / n);
+        hipLaunchKernelGGL(lnt_k_tw4, dim3((n + 255) / 256), dim3(256), 0, c->stream, omn, pl.C, pl.col.B1,
+                           pl.col.B2, n, c->tw4);
+        hipLaunchKernelGGL(lnt_k_tw4, dim3((n + 255) / 256), dim3(256), 0, c->stream, gl_inv(omn), pl.C, pl.col.B1,
+                           pl.col.B2, n, c->itw4);
+    }
+    HIPCHK(c, hipGetLastError(), "table kernels launch");
+    LntParams &P = c->P;
+    P.x = view<u64>(c->x, n);
+    P.y = view<u64>(c->y, pl.small ? 0 : n);
+    P.w = view<const u64>(c->w, n);
+    P.uw = view<const u64>(c->uw, n);
+    P.tw4 = view<const u64>(c->tw4, pl.small ? 0 : n);
+    P.itw4 = view<const u64>(c->itw4, pl.small ? 0 : n);
+    P.twr = view<const u64>(c->twr, Lr);
+    P.itwr = view<const u64>(c->itwr, Lr);
+    P.twc = view<const u64>(c->twc, pl.small ? 0 : pl.R);
+    P.itwc = view<const u64>(c->itwc, pl.small ? 0 : pl.R);
+    P.p = c->p;
+    P.n = n;
+    P.log_n = pl.log_n;
+    P.C = pl.C;
+    P.R = pl.R;
+    P.levels = pl.levels;
+    return LNT_OK;
+}
+
+static int upload_digits(lnt_ctx *c, const uint64_t *d) {
+    const u32 n = c->pl.n;
+    std::vector<uint64_t> xd(n);
+    to_device_digits(c->wid, d, xd.data());
+    HIPCHK(c, hipMemcpyAsync(c->x, xd.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->stream), "hipMemcpy (x)");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    c->pending = false;
+    c->pristine = true;
+    c->set_digits.assign(d, d + n);
+    // value 0 (mod M_p) given as a nonzero vector: the reference keeps the all-ones form
+    std::vector<uint64_t> canon(n);
+    bool zero = to_canonical_digits(c->wid, xd.data(), false, canon.data());
+    bool all_zero = true;
+    for (u32 i = 0; i < n; i++)
+        if (d[i]) all_zero = false;
+    c->zero_ones = zero && !all_zero;
+    return LNT_OK;
+}
+
+// One or more iterations; large path: three launches each.
+static int run_iterations(lnt_ctx *c, uint32_t k) {
+    const Plan &pl = c->pl;
+    const bool ll = !c->prp;
+    if (pl.small) {
+        // bounded launches keep each kernel short
+        const uint32_t chunk = pl.n >= 2048 ? 256 : 1024;
+        while (k) {
+            uint32_t s = k < chunk ? k : chunk;
+            if (!lnt_launch_small(pl, c->P, s, ll && c->pending, ll, c->stream))
+                return fail(c, LNT_UNSUPPORTED, "no small-path kernel for this length");
+            c->pending = true;
+            k -= s;
+        }
+    } else {
+        for (uint32_t i = 0; i < k; i++) {
+            if (!lnt_launch_col_fwd(pl, c->P, ll && c->pending, c->stream) || !lnt_launch_row(pl, c->P, c->stream) ||
+                !lnt_launch_col_inv(pl, c->P, c->stream))
+                return fail(c, LNT_UNSUPPORTED, "no kernel for this row/column split");
+            c->pending = true;
+        }
+    }
+    HIPCHK(c, hipGetLastError(), "kernel launch");
+    c->pristine = false;
+    return LNT_OK;
+}
+
+static int make_context(int device, uint32_t p, uint64_t n, bool prp, uint32_t force_lc, lnt_ctx **out) {
+    lnt_ctx *c = new lnt_ctx;
+    c->device = device;
+    c->p = p;
+    c->prp = prp;
+    std::string why = make_plan(p, n, c->pl, force_lc);
+    if (!why.empty()) {
+        delete c;
+        return fail(nullptr, LNT_UNSUPPORTED, why);
+    }
+    auto bail = [&](int code) {
+        std::string m = c->err;
+        free_buffers(c);
+        if (c->ev_mark) (void)hipEventDestroy(c->ev_mark);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(nullptr, code, m);
+    };
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        c->err = std::string("hipSetDevice failed (") + hipGetErrorString(e) + ")";
+        return bail(LNT_DEVICE_ERROR);
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev_mark) != hipSuccess) {
+        c->err = "hipStreamCreate / hipEventCreate failed";
+        return bail(LNT_DEVICE_ERROR);
+    }
+    widths(p, (uint32_t)n, c->wid);
+    int rc = build_tables(c);
+    if (rc != LNT_OK) return bail(rc);
+    std::vector<uint64_t> d(n, 0);
+    d[0] = prp ? 3 : 4;  // lucas_prp.cu:129
+    rc = upload_digits(c, d.data());
+    if (rc != LNT_OK) return bail(rc);
+    if (hipEventRecord(c->ev_mark, c->stream) != hipSuccess) {
+        c->err = "hipEventRecord failed";
+        return bail(LNT_DEVICE_ERROR);
+    }
+    *out = c;
+    return LNT_OK;
+}
+
+static int read_digits(lnt_ctx *c, uint64_t *d) {
+    const u32 n = c->pl.n;
+    if (c->pristine) {
+        std::memcpy(d, c->set_digits.data(), (size_t)n * 8);
+        return LNT_OK;
+    }
+    std::vector<uint64_t> xd(n);
+    HIPCHK(c, hipMemcpyAsync(xd.data(), c->x, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpy (x)");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    bool zero = to_canonical_digits(c->wid, xd.data(), !c->prp && c->pending, d);
+    if (zero && c->prp && c->zero_ones)
+        for (u32 i = 0; i < n; i++) d[i] = (1ull << c->wid[i]) - 1;
+    return LNT_OK;
+}
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+int lnt_create(int device, uint32_t p, const uint32_t *factors, int nfactors, int mode, lnt_ctx **out) {
+    if (!out) return fail(nullptr, LNT_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (mode != LNT_MODE_LL && mode != LNT_MODE_PRP) return fail(nullptr, LNT_INVALID_ARGUMENT, "bad mode");
+    uint64_t n = 1;
+    if (nfactors <= 0 || !factors) n = 1ull << secured_log2_length(p);
+    else
+        for (int i = 0; i < nfactors; i++) {
+            if (factors[i] == 0) return fail(nullptr, LNT_INVALID_ARGUMENT, "zero NTT factor");
+            n *= factors[i];
+            if (n > (1ull << 32)) return fail(nullptr, LNT_UNSUPPORTED, "NTT length too large");
+        }
+    return make_context(device, p, n, mode == LNT_MODE_PRP, 0, out);
+}
+
+int lnt_set_digits(lnt_ctx *c, const uint64_t *d, size_t n) {
+    if (!c || !d) return fail(c, LNT_INVALID_ARGUMENT, "NULL argument");
+    if (n != c->pl.n) return fail(c, LNT_INVALID_ARGUMENT, "digit count does not match the NTT length");
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    return upload_digits(c, d);
+}
+
+int lnt_iterate(lnt_ctx *c, uint32_t k) {
+    if (!c) return fail(c, LNT_INVALID_ARGUMENT, "NULL context");
+    if (k == 0) return LNT_OK;
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    return run_iterations(c, k);
+}
+
+int lnt_get_digits(lnt_ctx *c, uint64_t *d, size_t n) {
+    if (!c || !d) return fail(c, LNT_INVALID_ARGUMENT, "NULL argument");
+    if (n != c->pl.n) return fail(c, LNT_INVALID_ARGUMENT, "digit count does not match the NTT length");
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    return read_digits(c, d);
+}
+
+int lnt_res64(lnt_ctx *c, uint64_t *out) {
+    if (!c || !out) return fail(c, LNT_INVALID_ARGUMENT, "NULL argument");
+    std::vector<uint64_t> d(c->pl.n);
+    int rc = lnt_get_digits(c, d.data(), d.size());
+    if (rc) return rc;
+    bool ones = true;
+    for (u32 i = 0; i < c->pl.n && ones; i++)
+        if (d[i] != (1ull << c->wid[i]) - 1) ones = false;
+    *out = ones ? 0 : res64(c->wid, d.data());
+    return LNT_OK;
+}
+
+int lnt_sync(lnt_ctx *c, float *elapsed_ms) {
+    if (!c) return fail(c, LNT_INVALID_ARGUMENT, "NULL context");
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    hipEvent_t now;
+    HIPCHK(c, hipEventCreate(&now), "hipEventCreate");
+    HIPCHK(c, hipEventRecord(now, c->stream), "hipEventRecord");
+    HIPCHK(c, hipEventSynchronize(now), "hipEventSynchronize");
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev_mark, now), "hipEventElapsedTime");
+    (void)hipEventDestroy(c->ev_mark);
+    c->ev_mark = now;
+    if (elapsed_ms) *elapsed_ms = ms;
+    return LNT_OK;
+}
+
+void lnt_destroy(lnt_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_buffers(c);
+    if (c->ev_mark) (void)hipEventDestroy(c->ev_mark);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *lnt_last_error(const lnt_ctx *c) {
+    if (c) return c->err.c_str();
+    std::lock_guard<std::mutex> g(g_err_mu);
+    return g_err.c_str();
+}
+
+uint32_t lnt_secured_log2_length(uint32_t p) { return secured_log2_length(p); }
+
+uint32_t lnt_length(const lnt_ctx *c) { return c ? c->pl.n : 0; }
+
+static std::string describe(const Plan &pl) {
+    char b[256];
+    if (pl.small) {
+        std::snprintf(b, sizeof b, "LNT_1WG;lnt_k_small<%u,%u>", pl.row.B1, pl.row.B2);
+    } else {
+        std::snprintf(b, sizeof b, "LNT_2PASS_%ux%u;lnt_k_col_fwd<%u,%u,%u>;lnt_k_row<%u,%u>;lnt_k_col_inv<%u,%u,%u>",
+                      pl.R, pl.C, pl.col.B1, pl.col.B2, pl.K, pl.row.B1, pl.row.B2, pl.col.B1, pl.col.B2, pl.K);
+    }
+    return b;
+}
+
+int lnt_describe(const lnt_ctx *c, char *buf, size_t len) {
+    if (!c || !buf || !len) return LNT_INVALID_ARGUMENT;
+    std::string s = describe(c->pl);
+    std::snprintf(buf, len, "%s", s.c_str());
+    return s.size() < len ? LNT_OK : LNT_INVALID_ARGUMENT;
+}
+
+int lnt_profile(lnt_ctx *c, uint32_t iters, char *names, size_t names_len, float *ms, int cap) {
+    if (!c || !iters) return fail(c, LNT_INVALID_ARGUMENT, "bad argument");
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    const Plan &pl = c->pl;
+    const bool ll = !c->prp;
+    const int nk = pl.small ? 1 : 3;
+    std::vector<hipEvent_t> ev((size_t)nk * iters + 1);
+    for (auto &e : ev) HIPCHK(c, hipEventCreate(&e), "hipEventCreate");
+    HIPCHK(c, hipEventRecord(ev[0], c->stream), "hipEventRecord");
+    size_t ei = 1;
+    for (uint32_t i = 0; i < iters; i++) {
+        if (pl.small) {
+            lnt_launch_small(pl, c->P, 1, ll && c->pending, ll, c->stream);
+            c->pending = true;
+            HIPCHK(c, hipEventRecord(ev[ei++], c->stream), "hipEventRecord");
+        } else {
+            lnt_launch_col_fwd(pl, c->P, ll && c->pending, c->stream);
+            c->pending = true;
+            HIPCHK(c, hipEventRecord(ev[ei++], c->stream), "hipEventRecord");
+            lnt_launch_row(pl, c->P, c->stream);
+            HIPCHK(c, hipEventRecord(ev[ei++], c->stream), "hipEventRecord");
+            lnt_launch_col_inv(pl, c->P, c->stream);
+            HIPCHK(c, hipEventRecord(ev[ei++], c->stream), "hipEventRecord");
+        }
+    }
+    c->pristine = false;
+    HIPCHK(c, hipGetLastError(), "kernel launch");
+    HIPCHK(c, hipEventSynchronize(ev.back()), "hipEventSynchronize");
+    std::vector<double> tot(nk, 0.0);
+    for (size_t i = 1; i < ev.size(); i++) {
+        float t = 0;
+        HIPCHK(c, hipEventElapsedTime(&t, ev[i - 1], ev[i]), "hipEventElapsedTime");
+        tot[(i - 1) % nk] += t;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    std::string d = describe(pl);
+    std::vector<std::string> parts;
+    size_t pos = 0;
+    while (true) {
+        size_t q = d.find(';', pos);
+        parts.push_back(d.substr(pos, q == std::string::npos ? std::string::npos : q - pos));
+        if (q == std::string::npos) break;
+        pos = q + 1;
+    }
+    std::string nm;
+    int cnt = 0;
+    for (int k = 0; k < nk && k < cap; k++) {
+        if (ms) ms[k] = (float)(tot[k] / iters);
+        if (k) nm += ";";
+        nm += parts[k + 1];
+        cnt++;
+    }
+    if (names && names_len) std::snprintf(names, names_len, "%s", nm.c_str());
+    return cnt;
+}
+
+int lnt_tune(lnt_ctx *c, int loops) {
+    if (!c || loops <= 0) return fail(c, LNT_INVALID_ARGUMENT, "bad argument");
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    if (c->pl.small) return LNT_OK;  // one configuration only
+    const u32 n = c->pl.n;
+    // current residue, exactly as the next iteration will see it
+    std::vector<uint64_t> start(n);
+    HIPCHK(c, hipMemcpyAsync(start.data(), c->x, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpy");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    const bool start_pending = c->pending;
+    struct Cand {
+        uint32_t lc;
+        float ms;
+        std::vector<uint64_t> out;
+    };
+    std::vector<Cand> cands;
+    for (uint32_t lc = 7; lc <= 12; lc++) {
+        uint32_t lr = c->pl.log_n - lc;
+        if (lr < 6 || lr > 12) continue;
+        lnt_ctx *t = nullptr;
+        if (make_context(c->device, c->p, n, c->prp, lc, &t) != LNT_OK) continue;
+        HIPCHK(c, hipMemcpyAsync(t->x, start.data(), (size_t)n * 8, hipMemcpyHostToDevice, t->stream), "hipMemcpy");
+        t->pending = start_pending;
+        t->pristine = false;
+        float ms = 0;
+        lnt_sync(t, nullptr);
+        if (run_iterations(t, (uint32_t)loops) != LNT_OK || lnt_sync(t, &ms) != LNT_OK) {
+            lnt_destroy(t);
+            continue;
+        }
+        Cand cd{lc, ms, std::vector<uint64_t>(n)};
+        read_digits(t, cd.out.data());
+        lnt_destroy(t);
+        cands.push_back(std::move(cd));
+    }
+    if (cands.empty()) return fail(c, LNT_DEVICE_ERROR, "tuner: no candidate ran");
+    // every candidate must agree with the others (the current split is among them)
+    for (auto &cd : cands)
+        if (cd.out != cands[0].out) return fail(c, LNT_DEVICE_ERROR, "tuner: candidates disagree on the residue");
+    const Cand *best = &cands[0];
+    for (auto &cd : cands)
+        if (cd.ms < best->ms) best = &cd;
+    uint32_t cur_lc = ilog2(c->pl.C);
+    if (best->lc == cur_lc) return LNT_OK;
+    // rebuild this context on the winning split, keeping the residue
+    lnt_ctx *nc = nullptr;
+    int rc = make_context(c->device, c->p, n, c->prp, best->lc, &nc);
+    if (rc != LNT_OK) return fail(c, rc, lnt_last_error(nullptr));
+    HIPCHK(c, hipMemcpyAsync(nc->x, start.data(), (size_t)n * 8, hipMemcpyHostToDevice, nc->stream), "hipMemcpy");
+    HIPCHK(c, hipStreamSynchronize(nc->stream), "hipStreamSynchronize");
+    std::swap(c->pl, nc->pl);
+    std::swap(c->x, nc->x);
+    std::swap(c->y, nc->y);
+    std::swap(c->w, nc->w);
+    std::swap(c->uw, nc->uw);
+    std::swap(c->tw4, nc->tw4);
+    std::swap(c->itw4, nc->itw4);
+    std::swap(c->twr, nc->twr);
+    std::swap(c->itwr, nc->itwr);
+    std::swap(c->twc, nc->twc);
+    std::swap(c->itwc, nc->itwc);
+    std::swap(c->P, nc->P);
+    lnt_destroy(nc);
+    return LNT_OK;
+}
+
+}  // extern "C"

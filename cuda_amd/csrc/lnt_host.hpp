@@ -1,0 +1,213 @@
+// Host-side logic of the engine: length selection, execution plan, carry bound,
+// conversions between the reference's canonical digits and the device representation.
+// Header-only; shared by the C ABI (lnt_api.hip) and the CPU emulation tests.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gl64.hpp"
+
+namespace lnt {
+
+typedef unsigned __int128 u128;
+typedef __int128 i128;
+
+// NttLengths::GetNttSecuredLength (ntt_lengths.cu:88-98) in exact integer arithmetic:
+// L = ceil(log2(p / 32)); w = ceil(p / 2^L); if (2^w - 1)^2 >= P / 2^(L+1): L++.
+// The double-precision original agrees for every p >= 33 (checked against the oracle on
+// a dense sweep in tests/test_host.py).
+inline uint32_t secured_log2_length(uint32_t p) {
+    if (p <= 32) return 0;
+    uint32_t L = 0;
+    while ((32ull << L) < p) L++;
+    uint64_t w = ((uint64_t)p + (1ull << L) - 1) >> L;
+    u128 m = (w >= 64) ? ~(u128)0 : (((u128)1 << w) - 1);
+    u128 lhs = m * m;
+    // m^2 >= P / 2^(L+1)  <=>  m^2 * 2^(L+1) >= P   (real division)
+    bool bump = (lhs >> (127 - (L + 1))) != 0 ? true : ((lhs << (L + 1)) >= (u128)GL_P);
+    if (bump) L++;
+    return L;
+}
+
+inline uint32_t ilog2(uint64_t v) {
+    uint32_t l = 0;
+    while ((1ull << l) < v) l++;
+    return l;
+}
+
+// Line shape for a line of 2^a words: B1 = 2^ceil(a/2), B2 = 2^floor(a/2).
+struct Line {
+    uint32_t B1 = 0, B2 = 0;
+};
+inline Line line_for(uint32_t a) {
+    Line l;
+    l.B1 = 1u << ((a + 1) / 2);
+    l.B2 = 1u << (a / 2);
+    return l;
+}
+
+struct Plan {
+    bool small = false;     // single-workgroup path (n <= 4096)
+    uint32_t n = 0, log_n = 0;
+    uint32_t C = 0, R = 0;  // large path: row length, row count
+    Line row, col;          // row line (also the small path's line), column line
+    uint32_t K = 0;         // columns per column tile
+    uint32_t G = 0;         // rows per row workgroup
+    uint32_t M = 0;         // carry halo (compile-time), >= levels
+    uint32_t levels = 0;    // carry levels
+};
+
+// Carry levels m such that digits stay bounded by D with 2 n D^2 < P/2 in steady state
+// (see the bound in lnt_kernels.hpp's header).  0 if no m <= 8 works.
+inline uint32_t carry_levels(uint32_t p, uint32_t n) {
+    const double wmin = std::floor((double)p / n), wmax = std::ceil((double)p / n);
+    const double half_p = std::ldexp(1.0, 63) * 0.999999;  // < P/2
+    for (uint32_t m = 1; m <= 8; m++) {
+        double D = std::ldexp(1.0, (int)wmax - 1) + 2;
+        bool ok = false;
+        for (int round = 0; round < 200; round++) {
+            double Mv = 2.0 * n * D * D * 1.0001;
+            if (Mv >= half_p) break;
+            double V = Mv + 2;
+            for (uint32_t l = 0; l < m; l++) {
+                double H = V / std::ldexp(1.0, (int)wmin) + 0.5;
+                V = std::ldexp(1.0, (int)wmax - 1) + H;
+            }
+            double Dn = std::ceil(V) + 1;
+            if (Dn <= D) {
+                ok = true;
+                break;
+            }
+            D = Dn;
+        }
+        if (ok) return m;
+    }
+    return 0;
+}
+
+// Row/column split of a large residue: log2 C and log2 R.
+inline bool split_for(uint32_t log_n, uint32_t &lc, uint32_t &lr) {
+    if (log_n < 13 || log_n > 24) return false;
+    lc = log_n >= 22 ? 12 : (log_n + 1) / 2;
+    lr = log_n - lc;
+    return lr <= 12 && lr >= 6;
+}
+
+// Build the plan; returns an empty string on success, else the reason.
+inline std::string make_plan(uint32_t p, uint64_t n, Plan &pl, uint32_t force_lc = 0) {
+    if (n < 8 || (n & (n - 1))) return "NTT length must be a power of two >= 8";
+    pl.n = (uint32_t)n;
+    pl.log_n = ilog2(n);
+    if (pl.log_n > 24) return "NTT lengths above 2^24 are not supported by this engine";
+    if ((uint64_t)p < 2 * n) return "exponent too small for this NTT length (needs p >= 2n)";
+    if ((uint64_t)p / n > 31) return "NTT length too small for this exponent (digits wider than 31 bits)";
+    pl.levels = carry_levels(p, (uint32_t)n);
+    if (pl.levels == 0) return "NTT length too small for this exponent (convolution would overflow)";
+    pl.M = pl.levels <= 4 ? 4 : 8;
+    if (pl.log_n <= 12) {
+        pl.small = true;
+        pl.row = line_for(pl.log_n);
+        pl.C = pl.n;
+        pl.R = 1;
+        return "";
+    }
+    uint32_t lc, lr;
+    if (!split_for(pl.log_n, lc, lr)) return "unsupported NTT length";
+    if (force_lc) {
+        lc = force_lc;
+        lr = pl.log_n - lc;
+        if (lc < 7 || lc > 12 || lr < 6 || lr > 12) return "unsupported row/column split";
+    }
+    pl.C = 1u << lc;
+    pl.R = 1u << lr;
+    pl.row = line_for(lc);
+    pl.col = line_for(lr);
+    pl.K = pl.R >= 4096 ? 4 : pl.R >= 2048 ? 8 : 16;
+    pl.G = 64 / pl.row.B2;
+    return "";
+}
+
+// Digit widths w_i = ceil(p(i+1)/n) - ceil(p i/n).
+inline void widths(uint32_t p, uint32_t n, std::vector<uint8_t> &w) {
+    w.resize(n);
+    uint32_t L = ilog2(n);
+    for (uint32_t i = 0; i < n; i++) {
+        uint64_t e0 = ((uint64_t)p * i + n - 1) >> L, e1 = ((uint64_t)p * (i + 1) + n - 1) >> L;
+        w[i] = (uint8_t)(e1 - e0);
+    }
+}
+
+// Canonical digits -> balanced signed digits (field-encoded), exact value preserved
+// mod 2^p - 1.  Any non-negative input digits are accepted.
+inline void to_device_digits(const std::vector<uint8_t> &w, const uint64_t *d, uint64_t *x) {
+    const size_t n = w.size();
+    std::vector<int64_t> b(n);
+    i128 carry = 0;
+    for (size_t i = 0; i < n; i++) b[i] = 0;
+    // first pass with the raw digits, later passes only push the wrapped carry
+    for (int pass = 0; pass < 64; pass++) {
+        for (size_t i = 0; i < n; i++) {
+            i128 t = (pass == 0 ? (i128)d[i] : (i128)b[i]) + carry;
+            int wi = w[i];
+            i128 hi = (t + ((i128)1 << (wi - 1))) >> wi;
+            b[i] = (int64_t)(t - (hi << wi));
+            carry = hi;
+            if (pass > 0 && carry == 0) break;
+        }
+        if (carry == 0) break;
+        if (carry >= -1 && carry <= 1) {  // fold a unit wrap into digit 0 (slack of one)
+            b[0] += (int64_t)carry;
+            carry = 0;
+            break;
+        }
+    }
+    for (size_t i = 0; i < n; i++) x[i] = gl_from_i64(b[i]);
+}
+
+// Device words (pre-carry signed values) -> canonical digits in [0, 2^w).  Returns true
+// if the value is 0 mod 2^p - 1.
+inline bool to_canonical_digits(const std::vector<uint8_t> &w, const uint64_t *x, bool sub2,
+                                uint64_t *d) {
+    const size_t n = w.size();
+    std::vector<int64_t> v(n);
+    for (size_t i = 0; i < n; i++) v[i] = gl_to_i64(x[i]);
+    if (sub2) v[0] -= 2;
+    i128 carry = 0;
+    bool first = true;
+    for (int pass = 0; pass < 1000; pass++) {
+        size_t i = 0;
+        for (; i < n; i++) {
+            i128 t = (first ? (i128)v[i] : (i128)d[i]) + carry;
+            int wi = w[i];
+            i128 q = t >> wi;  // floor
+            d[i] = (uint64_t)(t - (q << wi));
+            carry = q;
+            if (!first && carry == 0) break;
+        }
+        first = false;
+        if (carry == 0) break;
+    }
+    // all-ones is 2^p - 1 = 0 (mod 2^p - 1): canonical 0
+    bool all_ones = true, all_zero = true;
+    for (size_t i = 0; i < n; i++) {
+        if (d[i] != (1ull << w[i]) - 1) all_ones = false;
+        if (d[i] != 0) all_zero = false;
+    }
+    if (all_ones) std::memset(d, 0, n * sizeof(uint64_t));
+    return all_ones || all_zero;
+}
+
+inline uint64_t res64(const std::vector<uint8_t> &w, const uint64_t *d) {
+    uint64_t r = 0;
+    uint32_t e = 0;
+    for (size_t i = 0; i < w.size() && e < 64; i++) {
+        r += d[i] << e;
+        e += w[i];
+    }
+    return r;
+}
+
+}  // namespace lnt

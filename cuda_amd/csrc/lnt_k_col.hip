@@ -1,0 +1,35 @@
+// Instantiations of the column kernels (column lengths 64 .. 4096).
+#include "lnt_launch.hpp"
+
+template <int B1, int B2, int K>
+static bool go_col_fwd(const lnt::Plan &pl, const LntParams &P, bool sub2, hipStream_t s) {
+    dim3 grid(pl.C / K), block(K * B2);
+    if (pl.M == 4)
+        hipLaunchKernelGGL((lnt_k_col_fwd<B1, B2, K, 4>), grid, block, 0, s, P, (int)sub2);
+    else
+        hipLaunchKernelGGL((lnt_k_col_fwd<B1, B2, K, 8>), grid, block, 0, s, P, (int)sub2);
+    return true;
+}
+
+template <int B1, int B2, int K>
+static bool go_col_inv(const lnt::Plan &pl, const LntParams &P, hipStream_t s) {
+    dim3 grid(pl.C / K), block(K * B2);
+    hipLaunchKernelGGL((lnt_k_col_inv<B1, B2, K>), grid, block, 0, s, P);
+    return true;
+}
+
+#define COL_SHAPES(X) X(8, 8, 16) X(16, 8, 16) X(16, 16, 16) X(32, 16, 16) X(32, 32, 16) X(64, 32, 8) X(64, 64, 4)
+
+bool lnt_launch_col_fwd(const lnt::Plan &pl, const LntParams &P, bool sub2, hipStream_t s) {
+#define CF(b1, b2, k) if (pl.col.B1 == b1 && pl.col.B2 == b2 && pl.K == k) return go_col_fwd<b1, b2, k>(pl, P, sub2, s);
+    COL_SHAPES(CF)
+#undef CF
+    return false;
+}
+
+bool lnt_launch_col_inv(const lnt::Plan &pl, const LntParams &P, hipStream_t s) {
+#define CI(b1, b2, k) if (pl.col.B1 == b1 && pl.col.B2 == b2 && pl.K == k) return go_col_inv<b1, b2, k>(pl, P, s);
+    COL_SHAPES(CI)
+#undef CI
+    return false;
+}

@@ -1,0 +1,19 @@
+// Instantiations of the single-workgroup kernel for n = 8 .. 4096.
+#include "lnt_launch.hpp"
+
+template <int B1, int B2>
+static bool go_small(const lnt::Plan &pl, const LntParams &P, u32 iters, bool sub2_first, bool ll, hipStream_t s) {
+    dim3 grid(1), block(B2);
+    if (pl.M == 4)
+        hipLaunchKernelGGL((lnt_k_small<B1, B2, 4>), grid, block, 0, s, P, iters, (int)sub2_first, (int)ll);
+    else
+        hipLaunchKernelGGL((lnt_k_small<B1, B2, 8>), grid, block, 0, s, P, iters, (int)sub2_first, (int)ll);
+    return true;
+}
+
+bool lnt_launch_small(const lnt::Plan &pl, const LntParams &P, u32 iters, bool sub2_first, bool ll, hipStream_t s) {
+#define SM(b1, b2) if (pl.row.B1 == b1 && pl.row.B2 == b2) return go_small<b1, b2>(pl, P, iters, sub2_first, ll, s);
+    SM(4, 2) SM(4, 4) SM(8, 4) SM(8, 8) SM(16, 8) SM(16, 16) SM(32, 16) SM(32, 32) SM(64, 32) SM(64, 64)
+#undef SM
+    return false;
+}

@@ -1,0 +1,362 @@
+// Kernel bodies for one LL/PRP iteration, x <- x^2 (- 2) mod 2^p - 1, as an
+// irrational-base weighted NTT over GF(2^64 - 2^32 + 1).
+//
+// Residue layout (n = R * C words, digit i = c + C * r): natural digit order, each word a
+// field element that encodes a signed integer |v| < P/2.  Between iterations the words
+// hold the *pre-carry* convolution outputs of the last squaring; the carry (and the LL
+// "-2") of iteration k is applied at the start of iteration k+1, fused into the first
+// kernel.  Readout normalises on the host.
+//
+// Transform decomposition (a "line" of L = B1 * B2 words, B1 >= B2, both <= 64):
+//   phase 1: thread t < B2 holds words t + B2*j (j < B1), a B1-point register DFT,
+//            then the inter-block twiddle omega_L^(t * k2);
+//   phase 2: after an LDS transpose, B2-point register DFTs over t.
+//   Output X[B1*k1 + k2].  All roots of order <= 64 are powers of two, so the only
+//   general twiddle inside a line is the one between the phases.
+// Large residues (n >= 2^13) are a 2-D split: C-word rows (contiguous) and R-word
+// columns (stride C), with the 4-step twiddle omega_n^(c * kcol) after the column pass.
+//
+// Per iteration (large path), state x and work buffer y (the carry of one column tile
+// reads halo digits of the neighbouring tile, so col_fwd must not write x):
+//   col_fwd : x -> y  carry(+ -2) of the previous pre-carry values, weight, column NTT,
+//                     4-step twiddle
+//   row_sq  : y -> y  row NTT, pointwise square, inverse row NTT
+//   col_inv : y -> x  inverse 4-step tw, inverse column NTT, unweight (includes 1/n)
+// These play the roles of the reference's r2b + m2 + w_fftc_tw (kernels/r2b.cu:25-50,
+// kernels/m2.cu:23-41, kernels/w_fftc_tw__r2_8to4k.cu:20-46), fftr_sqr_ifftr
+// (kernels/fftr_sqr_ifftr__r2_8to4k.cu:20-56) and tw_ifftc_uw
+// (kernels/tw_ifftc_uw__r2_8to4k.cu:20-46).  Small residues (n <= 4096) run the whole
+// iteration loop inside one workgroup (the role of w_fft_sqr_ifft_uw__*,
+// kernels/w_fft_sqr_ifft_uw__r2_8to4k.cu:20-55, plus r2b and m2).
+//
+// Carry: "m-level" carry-save normalisation.  Each level splits every value into a
+// balanced remainder |lo| <= 2^(w-1) and a carry hi, and adds the carry of the previous
+// digit.  After m levels digit j depends only on the pre-carry values j-m..j, so a
+// workgroup that owns a run of digits needs just m "halo" values before the run, read
+// from memory - no inter-workgroup ordering.  The value sum v_j 2^e_j mod 2^p - 1 is
+// preserved exactly (the carry out of digit n-1 enters digit 0 with weight 2^p = 1).
+#pragma once
+#include "dft.hpp"
+#include "simt.hpp"
+
+struct LntParams {
+    GView<u64> x;            // n words: residue state (pre-carry values between iterations)
+    GView<u64> y;            // n words: transform-domain work buffer
+    GView<const u64> w, uw;  // IBDWT weights, unweights * n^-1 (natural digit order)
+    GView<const u64> tw4, itw4;  // 4-step twiddles (large path), natural slots
+    GView<const u64> twr, itwr;  // row-line twiddles (C words)
+    GView<const u64> twc, itwc;  // column-line twiddles (R words)
+    u32 p, n, log_n, C, R;
+    u32 levels;              // carry levels m
+};
+
+// e_g = ceil(p g / n) for 0 <= g <= n (bit offset of digit g).
+GL_HD u32 lnt_digit_offset(u32 p, u32 log_n, u64 g) {
+    return (u32)(((u64)p * g + ((1ull << log_n) - 1)) >> log_n);
+}
+
+// Balanced split of v by width w: v = lo + hi * 2^w, -2^(w-1) <= lo < 2^(w-1).
+GL_HD int64_t lnt_split(int64_t v, int w, int64_t &lo) {
+    int64_t hi = (v + (int64_t(1) << (w - 1))) >> w;
+    lo = (int64_t)((u64)v - ((u64)hi << w));
+    return hi;
+}
+
+// m-level carry over a run: v[0..M) are the M digits before the run (halo), v[M..M+K)
+// the run itself, starting at digit g0.  On return v[M..M+K) hold final digits.
+template <int M, int K>
+GL_HD void lnt_carry_run(int64_t (&v)[M + K], u32 p, u32 log_n, u64 g0, u32 levels,
+                         bool sub2) {
+    const u64 n = 1ull << log_n;
+    int w[M + K];
+    u64 g = (g0 + n - M) & (n - 1);
+    u32 e = lnt_digit_offset(p, log_n, g);
+#pragma unroll
+    for (int j = 0; j < M + K; j++) {
+        u64 g1 = g + 1;
+        u32 e1 = lnt_digit_offset(p, log_n, g1);
+        w[j] = (int)(e1 - e);
+        e = e1;
+        g = g1;
+        if (g == n) {
+            g = 0;
+            e = 0;
+        }
+    }
+    if (sub2 && g0 == 0) v[M] -= 2;
+    for (u32 l = 0; l < levels; l++) {
+        int64_t hprev = 0;
+#pragma unroll
+        for (int j = 0; j < M + K; j++) {
+            int64_t lo;
+            int64_t h = lnt_split(v[j], w[j], lo);
+            v[j] = lo + hprev;
+            hprev = h;
+        }
+    }
+}
+
+constexpr int LNT_PAD = 1;
+
+// ------------------------------------------------------------------ column forward
+// grid: C / K workgroups (column tile c0 = bid * K), block: K * B2 threads.
+// LDS words: max(R * K, B1 * (B2 + PAD) * K).
+template <int B1, int B2, int K, int M>
+LNT_KFN void lnt_col_fwd_body(int tid, int bid, LView lds, const LntParams &P, bool sub2) {
+    constexpr int T = K * B2;
+    constexpr int RR = B1 * B2;
+    constexpr int RP = B1 / B2;
+    constexpr int S = B2 + LNT_PAD;
+    const u64 C = P.C;
+    const u64 n = P.n;
+    const u64 c0 = (u64)bid * K;
+
+    // 1. carry + (-2): one thread per row run of K digits (plus M halo digits)
+    for (int r = tid; r < RR; r += T) {
+        int64_t v[M + K];
+        const u64 g0 = C * r + c0;
+#pragma unroll
+        for (int j = 0; j < M + K; j++) {
+            u64 g = (g0 + n - M + j) & (n - 1);
+            v[j] = gl_to_i64(P.x[g]);
+        }
+        lnt_carry_run<M, K>(v, P.p, P.log_n, g0, P.levels, sub2);
+#pragma unroll
+        for (int c = 0; c < K; c++) lds[(u32)(r * K + c)] = gl_from_i64(v[M + c]);
+    }
+    LNT_SYNC();
+
+    // 2. phase 1: weight, B1-point DFT down the column, inter-block twiddle
+    const int cl = tid % K, t = tid / K;
+    const u64 c = c0 + cl;
+    u64 a[B1];
+#pragma unroll
+    for (int j = 0; j < B1; j++) {
+        const u64 r = t + B2 * j;
+        a[j] = gl_mul(lds[(u32)(r * K + cl)], P.w[c + C * r]);
+    }
+    dft_fwd<B1>(a);
+    if (B2 > 1) {
+#pragma unroll
+        for (int rho = 1; rho < B1; rho++) a[rho] = gl_mul(a[rho], P.twc[(u64)rho * B2 + t]);
+    }
+    LNT_SYNC();
+#pragma unroll
+    for (int rho = 0; rho < B1; rho++)
+        lds[(u32)((brev_c<B1>(rho) * S + t) * K + cl)] = a[rho];
+    LNT_SYNC();
+
+    // 3. phase 2: B2-point DFTs, 4-step twiddle, store (row r' = t*B1 + q*B2 + rho')
+#pragma unroll
+    for (int q = 0; q < RP; q++) {
+        const int k2 = t * RP + q;
+#pragma unroll
+        for (int i1 = 0; i1 < B2; i1++) a[q * B2 + i1] = lds[(u32)((k2 * S + i1) * K + cl)];
+        dft_fwd<B2>(a + q * B2);
+    }
+#pragma unroll
+    for (int i = 0; i < B1; i++) {
+        const u64 idx = c + C * ((u64)t * B1 + i);
+        P.y[idx] = gl_mul(a[i], P.tw4[idx]);
+    }
+}
+
+// ------------------------------------------------------------------ column inverse
+template <int B1, int B2, int K>
+LNT_KFN void lnt_col_inv_body(int tid, int bid, LView lds, const LntParams &P) {
+    constexpr int RP = B1 / B2;
+    constexpr int S = B2 + LNT_PAD;
+    const u64 C = P.C;
+    const int cl = tid % K, t = tid / K;
+    const u64 c = (u64)bid * K + cl;
+    u64 a[B1];
+#pragma unroll
+    for (int i = 0; i < B1; i++) {
+        const u64 idx = c + C * ((u64)t * B1 + i);
+        a[i] = gl_mul(P.y[idx], P.itw4[idx]);
+    }
+#pragma unroll
+    for (int q = 0; q < RP; q++) {
+        dft_inv<B2>(a + q * B2);
+        const int k2 = t * RP + q;
+#pragma unroll
+        for (int i1 = 0; i1 < B2; i1++) lds[(u32)((k2 * S + i1) * K + cl)] = a[q * B2 + i1];
+    }
+    LNT_SYNC();
+#pragma unroll
+    for (int rho = 0; rho < B1; rho++) {
+        u64 v = lds[(u32)((brev_c<B1>(rho) * S + t) * K + cl)];
+        a[rho] = (B2 > 1 && rho > 0) ? gl_mul(v, P.itwc[(u64)rho * B2 + t]) : v;
+    }
+    dft_inv<B1>(a);
+#pragma unroll
+    for (int j = 0; j < B1; j++) {
+        const u64 idx = c + C * ((u64)t + (u64)B2 * j);
+        P.x[idx] = gl_mul(a[j], P.uw[idx]);
+    }
+}
+
+// ------------------------------------------------------------------ row pass
+// grid: R / G workgroups, block: G * B2 threads (G rows per workgroup).
+// LDS words: G * B1 * (B2 + PAD).
+template <int B1, int B2, int G>
+LNT_KFN void lnt_row_body(int tid, int bid, LView lds, const LntParams &P) {
+    constexpr int RP = B1 / B2;
+    constexpr int S = B2 + LNT_PAD;
+    constexpr int LINE = B1 * S;
+    const int t = tid % B2, g = tid / B2;
+    const u64 base = (u64)P.C * ((u64)bid * G + g);
+    const u32 lb = (u32)(g * LINE);
+    u64 a[B1];
+#pragma unroll
+    for (int j = 0; j < B1; j++) a[j] = P.y[base + t + (u64)B2 * j];
+    dft_fwd<B1>(a);
+    if (B2 > 1) {
+#pragma unroll
+        for (int rho = 1; rho < B1; rho++) a[rho] = gl_mul(a[rho], P.twr[(u64)rho * B2 + t]);
+    }
+#pragma unroll
+    for (int rho = 0; rho < B1; rho++) lds[lb + (u32)(brev_c<B1>(rho) * S + t)] = a[rho];
+    LNT_SYNC();
+#pragma unroll
+    for (int q = 0; q < RP; q++) {
+        const int k2 = t * RP + q;
+#pragma unroll
+        for (int i1 = 0; i1 < B2; i1++) a[q * B2 + i1] = lds[lb + (u32)(k2 * S + i1)];
+        dft_fwd<B2>(a + q * B2);
+#pragma unroll
+        for (int i1 = 0; i1 < B2; i1++) a[q * B2 + i1] = gl_sqr(a[q * B2 + i1]);
+        dft_inv<B2>(a + q * B2);
+#pragma unroll
+        for (int i1 = 0; i1 < B2; i1++) lds[lb + (u32)(k2 * S + i1)] = a[q * B2 + i1];
+    }
+    LNT_SYNC();
+#pragma unroll
+    for (int rho = 0; rho < B1; rho++) {
+        u64 v = lds[lb + (u32)(brev_c<B1>(rho) * S + t)];
+        a[rho] = (B2 > 1 && rho > 0) ? gl_mul(v, P.itwr[(u64)rho * B2 + t]) : v;
+    }
+    dft_inv<B1>(a);
+#pragma unroll
+    for (int j = 0; j < B1; j++) P.y[base + t + (u64)B2 * j] = a[j];
+}
+
+// ------------------------------------------------------------------ small residues
+// One workgroup of B2 threads runs `iters` whole iterations on n = B1 * B2 words kept
+// in LDS (natural order, field elements).  LDS words: max(n, B1 * (B2 + PAD)).
+template <int B1, int B2, int M>
+LNT_KFN void lnt_small_body(int tid, LView lds, const LntParams &P, u32 iters, bool sub2_first,
+                            bool ll) {
+    constexpr int RP = B1 / B2;
+    constexpr int S = B2 + LNT_PAD;
+    constexpr u32 N = B1 * B2;
+    const int t = tid;
+#pragma unroll
+    for (int j = 0; j < B1; j++) lds[(u32)(t + B2 * j)] = P.x[(u64)t + B2 * j];
+    LNT_SYNC();
+    for (u32 it = 0; it < iters; it++) {
+        // tables re-read each iteration (from cache) instead of living in registers
+        const u64 *pw = P.w.p, *puw = P.uw.p, *ptw = P.twr.p, *pitw = P.itwr.p;
+        LNT_OPAQUE(pw);
+        LNT_OPAQUE(puw);
+        LNT_OPAQUE(ptw);
+        LNT_OPAQUE(pitw);
+        const GView<const u64> Tw{pw, P.w.n}, Tuw{puw, P.uw.n}, Ttw{ptw, P.twr.n}, Titw{pitw, P.itwr.n};
+        // carry over this thread's chunk [t*B1, (t+1)*B1), streamed in pieces of PK
+        // digits; each piece's halo is the M pre-carry values just before it.
+        {
+            constexpr int PK = B1 < 16 ? B1 : 16;
+            const u32 g0 = (u32)t * B1;
+            int64_t halo[M];
+#pragma unroll
+            for (int j = 0; j < M; j++) halo[j] = gl_to_i64(lds[(g0 + N - M + j) & (N - 1)]);
+            LNT_SYNC();  // every halo read before any chunk is overwritten
+            const bool sub2 = it == 0 ? sub2_first : ll;
+            for (int s = 0; s < B1 / PK; s++) {
+                int64_t v[M + PK];
+#pragma unroll
+                for (int j = 0; j < M; j++) v[j] = halo[j];
+#pragma unroll
+                for (int j = 0; j < PK; j++) v[M + j] = gl_to_i64(lds[g0 + s * PK + j]);
+#pragma unroll
+                for (int j = 0; j < M; j++) halo[j] = v[PK + j];
+                lnt_carry_run<M, PK>(v, P.p, P.log_n, g0 + s * PK, P.levels, sub2);
+#pragma unroll
+                for (int j = 0; j < PK; j++) lds[g0 + s * PK + j] = gl_from_i64(v[M + j]);
+            }
+            LNT_SYNC();
+        }
+        u64 a[B1];
+#pragma unroll
+        for (int j = 0; j < B1; j++) {
+            const u32 i = t + B2 * j;
+            a[j] = gl_mul(lds[i], Tw[i]);
+        }
+        dft_fwd<B1>(a);
+        if (B2 > 1) {
+#pragma unroll
+            for (int rho = 1; rho < B1; rho++) a[rho] = gl_mul(a[rho], Ttw[(u64)rho * B2 + t]);
+        }
+        LNT_SYNC();
+#pragma unroll
+        for (int rho = 0; rho < B1; rho++) lds[(u32)(brev_c<B1>(rho) * S + t)] = a[rho];
+        LNT_SYNC();
+#pragma unroll
+        for (int q = 0; q < RP; q++) {
+            const int k2 = t * RP + q;
+#pragma unroll
+            for (int i1 = 0; i1 < B2; i1++) a[q * B2 + i1] = lds[(u32)(k2 * S + i1)];
+            dft_fwd<B2>(a + q * B2);
+#pragma unroll
+            for (int i1 = 0; i1 < B2; i1++) a[q * B2 + i1] = gl_sqr(a[q * B2 + i1]);
+            dft_inv<B2>(a + q * B2);
+#pragma unroll
+            for (int i1 = 0; i1 < B2; i1++) lds[(u32)(k2 * S + i1)] = a[q * B2 + i1];
+        }
+        LNT_SYNC();
+#pragma unroll
+        for (int rho = 0; rho < B1; rho++) {
+            u64 v = lds[(u32)(brev_c<B1>(rho) * S + t)];
+            a[rho] = (B2 > 1 && rho > 0) ? gl_mul(v, Titw[(u64)rho * B2 + t]) : v;
+        }
+        dft_inv<B1>(a);
+        LNT_SYNC();
+#pragma unroll
+        for (int j = 0; j < B1; j++) {
+            const u32 i = t + B2 * j;
+            lds[i] = gl_mul(a[j], Tuw[i]);
+        }
+        LNT_SYNC();
+    }
+#pragma unroll
+    for (int j = 0; j < B1; j++) P.x[(u64)t + B2 * j] = lds[(u32)(t + B2 * j)];
+}
+
+// ------------------------------------------------------------------ tables
+// Twiddles of a line of L = B1 * B2 words: tw[rho * B2 + t] = omega_L^(t * brev_B1(rho)).
+GL_HD u64 lnt_line_tw(u64 omega_L, u32 B1, u32 B2, u32 idx) {
+    u32 rho = idx / B2, t = idx % B2;
+    u32 br = 0;
+    for (u32 b = 1, v = rho; b < B1; b <<= 1, v >>= 1) br = (br << 1) | (v & 1);
+    return gl_pow(omega_L, (u64)t * br);
+}
+
+// 4-step twiddle for slot idx = c + C * r', r' = t*B1 + q*B2 + rho' (column line B1 x B2):
+// omega_n^(c * kcol), kcol = B1 * brev_B2(rho') + t*(B1/B2) + q.
+GL_HD u64 lnt_tw4(u64 omega_n, u32 C, u32 B1, u32 B2, u64 idx) {
+    u32 c = (u32)(idx % C);
+    u32 rr = (u32)(idx / C);
+    u32 t = rr / B1, q = (rr % B1) / B2, rp = rr % B2;
+    u32 br = 0;
+    for (u32 b = 1, v = rp; b < B2; b <<= 1, v >>= 1) br = (br << 1) | (v & 1);
+    u64 kcol = (u64)B1 * br + (u64)t * (B1 / B2) + q;
+    return gl_pow(omega_n, (u64)c * kcol);
+}
+
+// IBDWT weight of digit i (common/pre_calculation.cu:23-36): B^(n - (p i mod n)) for
+// i > 0 and 1 for i = 0, with B = 7^((P-1)/192/n*5) (so B^n = 2).
+GL_HD u64 lnt_weight(u64 Bw, u32 p, u32 n, u32 i) {
+    if (i == 0) return 1;
+    u32 r = (u32)((u64)p * i) & (n - 1);
+    return gl_pow(Bw, (u64)(n - r));
+}

@@ -1,0 +1,200 @@
+"""Python binding of liblnt (include/lnt_api.h) over ctypes.
+
+The engine is the HIP library built in-tree (cuda_amd/liblnt.so).  There is no CPU
+fallback: if the library is missing or cannot reach a GPU, every call raises.
+
+`LucasPRP` mirrors the reference test driver (lucas_prp.cuh:15-39): one object per
+exponent and NTT length, seeded with 4 (LL) or 3 (PRP) (lucas_prp.cu:127-134), advanced
+one squaring per iteration (lucas_prp.cu:227-229), with the residue available as the
+reference's natural digit vector (lucas_prp.cu:254-263) and as a 64-bit residue.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Optional, Sequence, Tuple
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblnt.so")
+
+LNT_OK = 0
+LNT_UNSUPPORTED = 1
+LNT_DEVICE_ERROR = 2
+LNT_OUT_OF_MEMORY = 3
+LNT_INVALID_ARGUMENT = 4
+
+MODE_LL = 0
+MODE_PRP = 1
+
+# Every entry point declared in include/lnt_api.h.
+EXPORTS = (
+    "lnt_create", "lnt_set_digits", "lnt_iterate", "lnt_get_digits", "lnt_res64",
+    "lnt_sync", "lnt_tune", "lnt_destroy", "lnt_last_error", "lnt_secured_log2_length",
+    "lnt_length", "lnt_describe", "lnt_profile",
+)
+
+
+class LntError(RuntimeError):
+    """A failed liblnt call; `code` is the lnt_status value."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load liblnt.so (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is missing: build it with `make -C cuda_amd` or __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    vp, u32, u64p = ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)
+    lib.lnt_create.argtypes = [ctypes.c_int, u32, ctypes.POINTER(u32), ctypes.c_int, ctypes.c_int,
+                               ctypes.POINTER(vp)]
+    lib.lnt_set_digits.argtypes = [vp, u64p, ctypes.c_size_t]
+    lib.lnt_iterate.argtypes = [vp, u32]
+    lib.lnt_get_digits.argtypes = [vp, u64p, ctypes.c_size_t]
+    lib.lnt_res64.argtypes = [vp, u64p]
+    lib.lnt_sync.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    lib.lnt_tune.argtypes = [vp, ctypes.c_int]
+    lib.lnt_destroy.argtypes = [vp]
+    lib.lnt_destroy.restype = None
+    lib.lnt_last_error.argtypes = [vp]
+    lib.lnt_last_error.restype = ctypes.c_char_p
+    lib.lnt_secured_log2_length.argtypes = [u32]
+    lib.lnt_secured_log2_length.restype = u32
+    lib.lnt_length.argtypes = [vp]
+    lib.lnt_length.restype = u32
+    lib.lnt_describe.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+    lib.lnt_profile.argtypes = [vp, u32, ctypes.c_char_p, ctypes.c_size_t,
+                                ctypes.POINTER(ctypes.c_float), ctypes.c_int]
+    _lib = lib
+    return lib
+
+
+def secured_log2_length(p: int) -> int:
+    """log2 of the NTT length the reference picks for exponent p (ntt_lengths.cu:88-98)."""
+    return int(load().lnt_secured_log2_length(p))
+
+
+def parse_length(description: str) -> List[int]:
+    """Factors of an NTT length string such as "2048:2048", "4K" or "1M"
+    (NttLengths::StringToLength / StringToLengths, ntt_lengths.cu:32-75)."""
+    out = []
+    for part in description.split(":"):
+        part = part.strip()
+        factor = 1
+        if part[-1:].upper() == "K":
+            factor, part = 1024, part[:-1]
+        if part[-1:].upper() == "M":
+            factor, part = 1024 * 1024, part[:-1]
+        out.append(int(float(part)) * factor)
+    return out
+
+
+class LucasPRP:
+    """One Lucas-Lehmer (prp=False) or base-3 PRP (prp=True) test of M_p on the GPU."""
+
+    def __init__(self, p: int, length: Optional[str | int | Sequence[int]] = None,
+                 prp: bool = False, device: int = 0):
+        self._lib = load()
+        self.p = int(p)
+        self.prp = bool(prp)
+        if length is None:
+            factors: List[int] = []
+        elif isinstance(length, str):
+            factors = parse_length(length)
+        elif isinstance(length, int):
+            factors = [length]
+        else:
+            factors = [int(f) for f in length]
+        arr = (ctypes.c_uint32 * max(1, len(factors)))(*factors) if factors else None
+        h = ctypes.c_void_p()
+        rc = self._lib.lnt_create(device, self.p, arr, len(factors),
+                                  MODE_PRP if self.prp else MODE_LL, ctypes.byref(h))
+        if rc != LNT_OK:
+            raise LntError(rc, self._lib.lnt_last_error(None).decode())
+        self._h = h
+        self.n = int(self._lib.lnt_length(h))
+        self.iteration = 0
+
+    # -- helpers -----------------------------------------------------------------
+    def _check(self, rc: int) -> None:
+        if rc != LNT_OK:
+            raise LntError(rc, self._lib.lnt_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.lnt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- the hot path ----------------------------------------------------------------
+    def iterate(self, k: int = 1) -> None:
+        """Enqueue k iterations (asynchronous)."""
+        self._check(self._lib.lnt_iterate(self._h, int(k)))
+        self.iteration += int(k)
+
+    def sync(self) -> float:
+        """Wait; GPU milliseconds since the previous sync."""
+        ms = ctypes.c_float()
+        self._check(self._lib.lnt_sync(self._h, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def get_digits(self) -> List[int]:
+        buf = (ctypes.c_uint64 * self.n)()
+        self._check(self._lib.lnt_get_digits(self._h, buf, self.n))
+        return list(buf)
+
+    def get_digits_buffer(self):
+        buf = (ctypes.c_uint64 * self.n)()
+        self._check(self._lib.lnt_get_digits(self._h, buf, self.n))
+        return buf
+
+    def set_digits(self, digits: Sequence[int]) -> None:
+        if len(digits) != self.n:
+            raise ValueError(f"expected {self.n} digits, got {len(digits)}")
+        buf = (ctypes.c_uint64 * self.n)(*digits)
+        self._check(self._lib.lnt_set_digits(self._h, buf, self.n))
+
+    def res64(self) -> int:
+        out = ctypes.c_uint64()
+        self._check(self._lib.lnt_res64(self._h, ctypes.byref(out)))
+        return int(out.value)
+
+    def tune(self, loops: int = 50) -> None:
+        self._check(self._lib.lnt_tune(self._h, int(loops)))
+
+    def describe(self) -> List[str]:
+        buf = ctypes.create_string_buffer(1024)
+        self._check(self._lib.lnt_describe(self._h, buf, len(buf)))
+        return buf.value.decode().split(";")
+
+    def profile(self, iters: int) -> List[Tuple[str, float]]:
+        """Average GPU ms per launch of each kernel over `iters` iterations."""
+        names = ctypes.create_string_buffer(1024)
+        ms = (ctypes.c_float * 8)()
+        cnt = self._lib.lnt_profile(self._h, int(iters), names, len(names), ms, 8)
+        if cnt < 0:
+            raise LntError(-cnt, self._lib.lnt_last_error(self._h).decode())
+        if cnt == 0:
+            raise LntError(LNT_DEVICE_ERROR, self._lib.lnt_last_error(self._h).decode())
+        self.iteration += int(iters)
+        return list(zip(names.value.decode().split(";"), [float(ms[i]) for i in range(cnt)]))

@@ -1,0 +1,96 @@
+/*
+ * liblnt -- C ABI of the MI355X (gfx950) Lucas-Lehmer / PRP squaring engine.
+ *
+ * Plain C types only: pointers, sizes, integers.  No exceptions cross this boundary;
+ * every call returns an lnt_status and lnt_last_error() gives the message.  Host C++
+ * (cuda_amd/host) and Python (cuda_amd/lnt.py, ctypes) bind these entry points.
+ *
+ * The reference drives its hot path through C++ classes on one CUDA stream
+ * (LucasPRP, KernelPool, Kernel: lucas_prp.cuh:15-39, kernel.cuh:15-73); each entry
+ * point below names the reference call site whose role it takes.
+ *
+ * Residues cross the ABI as "natural digits": n unsigned words, digit i holding bits
+ * [ceil(p*i/n), ceil(p*(i+1)/n)) of the residue, exactly the order of the reference's
+ * .dat dump (lucas_prp.cu:254-263).
+ */
+#ifndef LNT_API_H
+#define LNT_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lnt_ctx lnt_ctx;
+
+typedef enum {
+    LNT_OK = 0,
+    LNT_UNSUPPORTED = 1, /* shape not supported (reference: Initialize() == false, process.cu:53) */
+    LNT_DEVICE_ERROR = 2,
+    LNT_OUT_OF_MEMORY = 3,
+    LNT_INVALID_ARGUMENT = 4
+} lnt_status;
+
+typedef enum { LNT_MODE_LL = 0, LNT_MODE_PRP = 1 } lnt_mode;
+
+/* Create a test context on `device` for M_p = 2^p - 1 with NTT length
+ * n = factors[0] * ... * factors[nfactors-1] (a power of two, 8 <= n <= 2^24).
+ * Allocates the residue and all tables and seeds the residue with 4 (LL) or 3 (PRP).
+ * Replaces LucasPRP::LucasPRP + LucasPRP::Initialize (lucas_prp.cu:24-111) together
+ * with the pool/kernel Initialize() table set-up (kernel.cu:91-153, 199-255) and
+ * InitializeTest (lucas_prp.cu:127-134).  The factor split is a hint only: the engine
+ * chooses its own row/column split (the role of KernelPool, kernel_pool.cu:17-141). */
+int lnt_create(int device, uint32_t p, const uint32_t *factors, int nfactors, int mode,
+               lnt_ctx **out);
+
+/* Replace the residue with n natural digits (LucasPRP::LoadResidue, lucas_prp.cu:326-346). */
+int lnt_set_digits(lnt_ctx *ctx, const uint64_t *natural_digits, size_t n);
+
+/* Enqueue k iterations (square, and -2 for LL) on the context's stream; asynchronous.
+ * Replaces k calls of LucasPRP::OneIteration (lucas_prp.cu:227-229). */
+int lnt_iterate(lnt_ctx *ctx, uint32_t k);
+
+/* Wait for the stream and copy out the canonical residue as n natural digits
+ * (the cudaMemcpy + DumpResidue order of lucas_prp.cu:164, 254-263). */
+int lnt_get_digits(lnt_ctx *ctx, uint64_t *natural_digits, size_t n);
+
+/* Low 64 bits of the canonical residue (new: the reference prints no res64). */
+int lnt_res64(lnt_ctx *ctx, uint64_t *out);
+
+/* Wait for the stream; *elapsed_ms = GPU time since the previous lnt_sync (or create).
+ * Replaces the event pair of LucasPRP::Run / DisplayPitStop (lucas_prp.cu:151-161, 232-234). */
+int lnt_sync(lnt_ctx *ctx, float *elapsed_ms);
+
+/* Time `loops` iterations on scratch state for every valid engine configuration of this
+ * length, validate each against the current configuration first, keep the fastest.
+ * The residue is preserved.  Replaces Process::GetBestPerformers / EvaluatePerformance
+ * (process.cu:45-149), which timed candidates without checking their results. */
+int lnt_tune(lnt_ctx *ctx, int loops);
+
+/* Free everything (LucasPRP::Finalize, lucas_prp.cu:113-125). */
+void lnt_destroy(lnt_ctx *ctx);
+
+/* Message of the last failure on ctx (or of the last failed lnt_create when ctx is NULL). */
+const char *lnt_last_error(const lnt_ctx *ctx);
+
+/* ---- host-side helpers (no device needed) ---- */
+
+/* log2 of the NTT length the reference selects for exponent p
+ * (NttLengths::GetNttSecuredLength, ntt_lengths.cu:88-98), in exact integer arithmetic. */
+uint32_t lnt_secured_log2_length(uint32_t p);
+
+/* Number of digits n of a context, and its description: "name;kernel0;kernel1;..." used
+ * for the stats file's Pool / Kernel<i> keys (lucas_prp.cu:311-313). */
+uint32_t lnt_length(const lnt_ctx *ctx);
+int lnt_describe(const lnt_ctx *ctx, char *buf, size_t buflen);
+
+/* Per-kernel average GPU durations (ms) over `iters` iterations, each kernel bracketed by
+ * events on the context's stream.  names/ms receive up to `cap` entries; returns count. */
+int lnt_profile(lnt_ctx *ctx, uint32_t iters, char *names, size_t names_len, float *ms, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LNT_API_H */

@@ -160,7 +160,7 @@ static void make_roots(uint32_t L, uint64_t **r, uint64_t **ir) {
 }
 
 /* fault kinds planted by tests; 0 = none */
-enum { F_NONE = 0, F_TWIDDLE_SIGN = 1, F_NO_WRAP = 2, F_NO_MINUS2 = 3, F_WEIGHT_INDEX = 4 };
+enum { F_NONE = 0, F_TWIDDLE_INDEX = 1, F_NO_WRAP = 2, F_NO_MINUS2 = 3, F_WEIGHT_INDEX = 4 };
 void lo_set_fault(lo_ctx *c, int kind) { c->fault = kind; }
 
 /* n0 = 1 selects the one-transform chain (FFT_1SMEM / FFT_GMEM: kernel_pool.cu:23-38);
@@ -216,8 +216,8 @@ static void ntt_fwd(uint64_t *a, uint32_t L, const uint64_t *roots, int fault) {
             for (uint32_t j = 0; j < half; j++) {
                 uint64_t u = a[s + j], v = a[s + j + half];
                 a[s + j] = lo_add(u, v);
-                uint64_t d = fault == F_TWIDDLE_SIGN ? lo_sub(v, u) : lo_sub(u, v);
-                a[s + j + half] = lo_mul(d, roots[j * step]);
+                uint32_t ti = j * step + (fault == F_TWIDDLE_INDEX && j > 0 ? 1 : 0);
+                a[s + j + half] = lo_mul(lo_sub(u, v), roots[ti]);
             }
     }
 }

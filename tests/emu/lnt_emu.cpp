@@ -1,0 +1,230 @@
+// CPU emulation of the engine's kernels (test infrastructure).
+//
+// Compiles cuda_amd/csrc/lnt_kernels.hpp with LNT_EMU: each workgroup's threads are
+// OpenMP threads, the workgroup barrier is an OpenMP barrier, and every global / LDS
+// index is bounds-checked.  It runs the same kernel bodies and the same host logic
+// (lnt_host.hpp) as the HIP build, so tests can check the decomposition, the carry and
+// the indexing against the oracle before the code runs on a GPU.
+#define LNT_EMU 1
+#include <omp.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "../../cuda_amd/csrc/lnt_host.hpp"
+#include "../../cuda_amd/csrc/lnt_kernels.hpp"
+
+using namespace lnt;
+
+struct Emu {
+    Plan pl;
+    uint32_t p;
+    bool prp, pending;
+    std::vector<uint64_t> x, y, w, uw, tw4, itw4, twr, itwr, twc, itwc;
+    std::vector<uint8_t> wid;
+    LntParams P;
+};
+
+template <class T>
+static GView<T> gv(std::vector<uint64_t> &v) {
+    return GView<T>{(T *)v.data(), (u64)v.size()};
+}
+
+static void line_tables(uint32_t L, Line ln, std::vector<uint64_t> &tw, std::vector<uint64_t> &itw) {
+    tw.resize(L);
+    itw.resize(L);
+    u64 om = gl_pow(554, (GL_P - 1) / L), iom = gl_inv(om);
+    for (uint32_t i = 0; i < L; i++) {
+        tw[i] = lnt_line_tw(om, ln.B1, ln.B2, i);
+        itw[i] = lnt_line_tw(iom, ln.B1, ln.B2, i);
+    }
+}
+
+extern "C" void *emu_create(uint32_t p, uint32_t n, int prp, uint32_t force_lc) {
+    Emu *e = new Emu;
+    e->p = p;
+    e->prp = prp;
+    std::string err = make_plan(p, n, e->pl, force_lc);
+    if (!err.empty()) {
+        std::fprintf(stderr, "emu_create: %s\n", err.c_str());
+        delete e;
+        return nullptr;
+    }
+    const Plan &pl = e->pl;
+    e->x.assign(n, 0);
+    e->y.assign(n, 0);
+    e->w.resize(n);
+    e->uw.resize(n);
+    u64 Bw = gl_pow(7, (GL_P - 1) / 192 / n * 5);
+    u64 ninv = gl_inv(n);
+    for (uint32_t i = 0; i < n; i++) {
+        e->w[i] = lnt_weight(Bw, p, n, i);
+        e->uw[i] = gl_mul(gl_inv(e->w[i]), ninv);
+    }
+    widths(p, n, e->wid);
+    line_tables(pl.small ? n : pl.C, pl.row, e->twr, e->itwr);
+    if (!pl.small) {
+        line_tables(pl.R, pl.col, e->twc, e->itwc);
+        e->tw4.resize(n);
+        e->itw4.resize(n);
+        u64 om = gl_pow(554, (GL_P - 1) / n), iom = gl_inv(om);
+        for (uint64_t i = 0; i < n; i++) {
+            e->tw4[i] = lnt_tw4(om, pl.C, pl.col.B1, pl.col.B2, i);
+            e->itw4[i] = lnt_tw4(iom, pl.C, pl.col.B1, pl.col.B2, i);
+        }
+    } else {
+        e->twc.assign(1, 1);
+        e->itwc.assign(1, 1);
+        e->tw4.assign(1, 1);
+        e->itw4.assign(1, 1);
+    }
+    LntParams &P = e->P;
+    P.x = gv<u64>(e->x);
+    P.y = gv<u64>(e->y);
+    P.w = gv<const u64>(e->w);
+    P.uw = gv<const u64>(e->uw);
+    P.tw4 = gv<const u64>(e->tw4);
+    P.itw4 = gv<const u64>(e->itw4);
+    P.twr = gv<const u64>(e->twr);
+    P.itwr = gv<const u64>(e->itwr);
+    P.twc = gv<const u64>(e->twc);
+    P.itwc = gv<const u64>(e->itwc);
+    P.p = p;
+    P.n = n;
+    P.log_n = pl.log_n;
+    P.C = pl.C;
+    P.R = pl.R;
+    P.levels = pl.levels;
+    std::vector<uint64_t> d(n, 0);
+    d[0] = prp ? 3 : 4;
+    to_device_digits(e->wid, d.data(), e->x.data());
+    e->pending = false;
+    return e;
+}
+
+extern "C" void emu_destroy(void *h) { delete (Emu *)h; }
+
+template <class F>
+static void run_wg(int threads, uint32_t lds_words, F body) {
+    std::vector<u64> lds(lds_words, 0xDEADBEEFDEADBEEFull);
+    LView L{lds.data(), lds_words};
+#pragma omp parallel num_threads(threads)
+    {
+        int tid = omp_get_thread_num();
+        body(tid, L);
+    }
+}
+
+template <int B1, int B2, int M>
+static void small_iter(Emu *e, uint32_t k) {
+    uint32_t lds = std::max<uint32_t>(B1 * B2, B1 * (B2 + LNT_PAD));
+    bool sub2 = !e->prp && e->pending;
+    run_wg(B2, lds, [&](int tid, LView L) { lnt_small_body<B1, B2, M>(tid, L, e->P, k, sub2, !e->prp); });
+}
+
+template <int B1, int B2, int K, int M>
+static void col_fwd(Emu *e, bool sub2) {
+    const Plan &pl = e->pl;
+    uint32_t lds = std::max<uint32_t>(pl.R * K, B1 * (B2 + LNT_PAD) * K);
+    for (uint32_t b = 0; b < pl.C / K; b++)
+        run_wg(K * B2, lds, [&](int tid, LView L) { lnt_col_fwd_body<B1, B2, K, M>(tid, b, L, e->P, sub2); });
+}
+
+template <int B1, int B2, int K>
+static void col_inv(Emu *e) {
+    const Plan &pl = e->pl;
+    uint32_t lds = B1 * (B2 + LNT_PAD) * K;
+    for (uint32_t b = 0; b < pl.C / K; b++)
+        run_wg(K * B2, lds, [&](int tid, LView L) { lnt_col_inv_body<B1, B2, K>(tid, b, L, e->P); });
+}
+
+template <int B1, int B2, int G>
+static void row_sq(Emu *e) {
+    const Plan &pl = e->pl;
+    uint32_t lds = G * B1 * (B2 + LNT_PAD);
+    for (uint32_t b = 0; b < pl.R / G; b++)
+        run_wg(G * B2, lds, [&](int tid, LView L) { lnt_row_body<B1, B2, G>(tid, b, L, e->P); });
+}
+
+// Dispatch tables for the emulated sizes (a subset of the HIP build's).
+static bool run_small(Emu *e, uint32_t k) {
+    const Plan &pl = e->pl;
+#define SM(b1, b2)                                                    \
+    if (pl.row.B1 == b1 && pl.row.B2 == b2) {                         \
+        if (pl.M == 4) small_iter<b1, b2, 4>(e, k);                   \
+        else small_iter<b1, b2, 8>(e, k);                             \
+        return true;                                                  \
+    }
+    SM(4, 2) SM(4, 4) SM(8, 4) SM(8, 8) SM(16, 8) SM(16, 16) SM(32, 16) SM(32, 32) SM(64, 32) SM(64, 64)
+#undef SM
+    return false;
+}
+
+static bool run_large_once(Emu *e, bool sub2) {
+    const Plan &pl = e->pl;
+    bool ok = false;
+#define CF(b1, b2, k)                                                            \
+    if (!ok && pl.col.B1 == b1 && pl.col.B2 == b2 && pl.K == k) {                \
+        if (pl.M == 4) col_fwd<b1, b2, k, 4>(e, sub2); else col_fwd<b1, b2, k, 8>(e, sub2); \
+        ok = true;                                                               \
+    }
+    CF(8, 8, 16) CF(16, 8, 16) CF(16, 16, 16) CF(32, 16, 16) CF(32, 32, 16) CF(64, 32, 8) CF(64, 64, 4)
+#undef CF
+    if (!ok) return false;
+    ok = false;
+#define RW(b1, b2)                                                        \
+    if (!ok && pl.row.B1 == b1 && pl.row.B2 == b2) {                      \
+        row_sq<b1, b2, 64 / b2>(e);                                       \
+        ok = true;                                                        \
+    }
+    RW(16, 8) RW(16, 16) RW(32, 16) RW(32, 32) RW(64, 32) RW(64, 64)
+#undef RW
+    if (!ok) return false;
+    ok = false;
+#define CI(b1, b2, k)                                                           \
+    if (!ok && pl.col.B1 == b1 && pl.col.B2 == b2 && pl.K == k) {               \
+        col_inv<b1, b2, k>(e);                                                  \
+        ok = true;                                                              \
+    }
+    CI(8, 8, 16) CI(16, 8, 16) CI(16, 16, 16) CI(32, 16, 16) CI(32, 32, 16) CI(64, 32, 8) CI(64, 64, 4)
+#undef CI
+    return ok;
+}
+
+extern "C" int emu_iterate(void *h, uint32_t k) {
+    Emu *e = (Emu *)h;
+    if (k == 0) return 0;
+    if (e->pl.small) {
+        if (!run_small(e, k)) return -1;
+    } else {
+        for (uint32_t i = 0; i < k; i++) {
+            bool sub2 = !e->prp && e->pending;
+            if (!run_large_once(e, sub2)) return -1;
+            e->pending = true;
+        }
+    }
+    e->pending = true;
+    return 0;
+}
+
+extern "C" void emu_get_digits(void *h, uint64_t *d) {
+    Emu *e = (Emu *)h;
+    to_canonical_digits(e->wid, e->x.data(), !e->prp && e->pending, d);
+}
+
+extern "C" uint64_t emu_res64(void *h) {
+    Emu *e = (Emu *)h;
+    std::vector<uint64_t> d(e->pl.n);
+    emu_get_digits(h, d.data());
+    return res64(e->wid, d.data());
+}
+
+extern "C" void emu_set_digits(void *h, const uint64_t *d) {
+    Emu *e = (Emu *)h;
+    to_device_digits(e->wid, d, e->x.data());
+    e->pending = false;
+}
+
+extern "C" uint32_t emu_levels(void *h) { return ((Emu *)h)->pl.levels; }
+extern "C" uint32_t emu_secured(uint32_t p) { return secured_log2_length(p); }

@@ -1,0 +1,113 @@
+#!/usr/bin/env python3
+"""Regenerate the golden res64 fixtures with the GMP big-integer oracle (oracle/gmp_ll.c).
+
+Each fixture is computed from the definition of the sequence, independently of any
+NTT: LL s_0 = 4, s_k = s_{k-1}^2 - 2 mod M_p; PRP s_0 = 3, s_k = s_{k-1}^2 mod M_p
+(lucas_prp.cu:127-134, kernels/m2.cu:23-41).  res64 = low 64 bits of s_k in [0, M_p).
+
+  python tests/golden/make_fixtures.py            # the fast fixtures (~1 minute)
+  python tests/golden/make_fixtures.py --long     # also the p = 70M / 140M runs (hours)
+  python tests/golden/make_fixtures.py --import-log 70000027 LL 20000 out.txt
+        # merge the "k res64" lines of `oracle/_build/gmp_ll 70000027 LL 20000 1000 > out.txt`
+        # (a long run made in the background) into the fixtures
+
+Output: tests/golden/res64.json
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+GMP = os.path.join(ROOT, "oracle", "_build", "gmp_ll")
+OUT = os.path.join(HERE, "res64.json")
+
+# Known Mersenne prime exponents (LL residue 0 after p-2 iterations, PRP residue 9).
+MERSENNE_PRIMES = [521, 607, 1279, 2203, 2281, 3217, 4253, 4423, 9689, 9941, 11213, 19937,
+                   21701, 23209, 44497, 86243]
+
+
+def is_prime(x: int) -> bool:
+    if x < 2:
+        return False
+    i = 2
+    while i * i <= x:
+        if x % i == 0:
+            return False
+        i += 1
+    return True
+
+
+def next_prime(x: int) -> int:
+    while not is_prime(x):
+        x += 1
+    return x
+
+
+# Prime exponents whose Mersenne numbers are composite (the residue is nonzero).
+COMPOSITES = [p for p in (523, 1009, 2003, 4001, 8191, 9697, 10007, 20011, 44501, 86249)
+              if is_prime(p)]
+
+
+def gmp(p: int, mode: str, iters: int, every: int) -> dict:
+    out = subprocess.run([GMP, str(p), mode, str(iters), str(every)], check=True,
+                         capture_output=True, text=True).stdout
+    res = {}
+    for line in out.strip().splitlines():
+        k, r = line.split()
+        res[k] = r
+    return res
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--long", action="store_true", help="also run the multi-hour p=70M/140M fixtures")
+    ap.add_argument("--import-log", nargs=4, metavar=("P", "MODE", "ITERS", "FILE"))
+    args = ap.parse_args()
+    if args.import_log:
+        p, mode, iters, path = args.import_log
+        doc = json.load(open(OUT))
+        res = dict(line.split() for line in open(path).read().strip().splitlines())
+        doc["cases"] = [c for c in doc["cases"] if not (c["p"] == int(p) and c["mode"] == mode
+                                                        and c["iters"] == int(iters))]
+        doc["cases"].append({"p": int(p), "mode": mode, "iters": int(iters), "res64": res})
+        with open(OUT, "w") as f:
+            json.dump(doc, f, indent=1, sort_keys=True)
+        print(f"imported {len(res)} checkpoints for p={p} {mode}")
+        return 0
+    if not os.path.exists(GMP):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True)
+    old = json.load(open(OUT)) if os.path.exists(OUT) else {"cases": []}
+    cases = []
+    for p in MERSENNE_PRIMES:
+        cases.append({"p": p, "mode": "LL", "iters": p - 2, "res64": gmp(p, "LL", p - 2, max(1, (p - 2) // 4))})
+        cases.append({"p": p, "mode": "PRP", "iters": p, "res64": gmp(p, "PRP", p, max(1, p // 4))})
+    for p in COMPOSITES:
+        cases.append({"p": p, "mode": "LL", "iters": p - 2, "res64": gmp(p, "LL", p - 2, max(1, (p - 2) // 4))})
+        cases.append({"p": p, "mode": "PRP", "iters": p, "res64": gmp(p, "PRP", p, max(1, p // 4))})
+    # the single-test parity workload: LL of M86243, res64 every 10,000 iterations
+    cases.append({"p": 86243, "mode": "LL", "iters": 86241, "res64": gmp(86243, "LL", 86241, 1000)})
+    long_cases = [(70000027, "LL", 20000, 1000), (140000041, "PRP", 10000, 1000)]
+    for p, mode, iters, every in long_cases:
+        if args.long:
+            cases.append({"p": p, "mode": mode, "iters": iters, "res64": gmp(p, mode, iters, every)})
+        else:
+            prev = [c for c in old["cases"] if c["p"] == p and c["mode"] == mode]
+            if prev:
+                cases.append(prev[0])
+    doc = {
+        "generator": "tests/golden/make_fixtures.py (oracle/gmp_ll.c, GMP mpz)",
+        "definition": "LL: s0=4, s=s^2-2 mod 2^p-1; PRP: s0=3, s=s^2 mod 2^p-1; res64 = low 64 bits "
+                      "of s_k in [0, 2^p-1); keys are iteration numbers k",
+        "cases": cases,
+    }
+    with open(OUT, "w") as f:
+        json.dump(doc, f, indent=1, sort_keys=True)
+    print(f"wrote {OUT}: {len(cases)} cases")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,165 @@
+"""Test helpers: the oracle (oracle/_build/liblnt_oracle.so), the CPU emulation of the
+kernels (tests/emu), golden fixtures.  Test infrastructure only."""
+import ctypes
+import json
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liblnt_oracle.so")
+EMU_SRC = os.path.join(HERE, "emu", "lnt_emu.cpp")
+EMU_SO = os.path.join(HERE, "emu", "_build", "libemu.so")
+GOLDEN = os.path.join(HERE, "golden", "res64.json")
+P_MOD = (1 << 64) - (1 << 32) + 1
+
+_oracle = None
+_emu = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True,
+                           capture_output=True)
+        L = ctypes.CDLL(ORACLE_SO)
+        u32, u64, vp = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p
+        for f in ("lo_add", "lo_sub", "lo_mul", "lo_pow"):
+            getattr(L, f).argtypes = [u64, u64]
+            getattr(L, f).restype = u64
+        L.lo_inv.argtypes = [u64]
+        L.lo_inv.restype = u64
+        L.lo_secured_length.argtypes = [u32]
+        L.lo_secured_length.restype = u32
+        L.lo_candidates.argtypes = [u32, ctypes.c_char_p, ctypes.c_int]
+        L.lo_tables.argtypes = [u32, u32, vp, vp, vp]
+        L.lo_create.argtypes = [u32, u32, u32, ctypes.c_int]
+        L.lo_create.restype = vp
+        L.lo_destroy.argtypes = [vp]
+        L.lo_iterate.argtypes = [vp, u32]
+        L.lo_set_digits.argtypes = [vp, vp]
+        L.lo_get_digits.argtypes = [vp, vp]
+        L.lo_get_widths.argtypes = [vp, vp]
+        L.lo_res64.argtypes = [vp]
+        L.lo_res64.restype = u64
+        L.lo_set_fault.argtypes = [vp, ctypes.c_int]
+        L.lo_n.argtypes = [vp]
+        L.lo_n.restype = u32
+        L.lo_threads.restype = ctypes.c_int
+        _oracle = L
+    return _oracle
+
+
+class OracleTest:
+    """The oracle's LucasPRP restatement (n0 = 1: one transform, else the 2-factor chain)."""
+
+    def __init__(self, p, n, prp=False, n0=1):
+        L = oracle()
+        self.L = L
+        self.n = n
+        self.h = L.lo_create(p, n0, n // n0, int(prp))
+
+    def __del__(self):
+        try:
+            self.L.lo_destroy(self.h)
+        except Exception:
+            pass
+
+    def iterate(self, k):
+        self.L.lo_iterate(self.h, k)
+
+    def res64(self):
+        return int(self.L.lo_res64(self.h))
+
+    def digits(self):
+        b = (ctypes.c_uint64 * self.n)()
+        self.L.lo_get_digits(self.h, b)
+        return list(b)
+
+    def set_digits(self, d):
+        b = (ctypes.c_uint64 * self.n)(*d)
+        self.L.lo_set_digits(self.h, b)
+
+    def widths(self):
+        b = (ctypes.c_uint8 * self.n)()
+        self.L.lo_get_widths(self.h, b)
+        return list(b)
+
+    def fault(self, kind):
+        self.L.lo_set_fault(self.h, kind)
+
+
+def emu():
+    """Build (if needed) and load the CPU emulation of the HIP kernels."""
+    global _emu
+    if _emu is None:
+        deps = [EMU_SRC] + [os.path.join(ROOT, "cuda_amd", "csrc", f)
+                            for f in os.listdir(os.path.join(ROOT, "cuda_amd", "csrc"))
+                            if f.endswith(".hpp")]
+        if not os.path.exists(EMU_SO) or any(os.path.getmtime(d) > os.path.getmtime(EMU_SO)
+                                             for d in deps):
+            os.makedirs(os.path.dirname(EMU_SO), exist_ok=True)
+            subprocess.run(["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-o",
+                            EMU_SO, EMU_SRC], check=True)
+        L = ctypes.CDLL(EMU_SO)
+        u32, vp = ctypes.c_uint32, ctypes.c_void_p
+        L.emu_create.argtypes = [u32, u32, ctypes.c_int, u32]
+        L.emu_create.restype = vp
+        L.emu_destroy.argtypes = [vp]
+        L.emu_iterate.argtypes = [vp, u32]
+        L.emu_get_digits.argtypes = [vp, vp]
+        L.emu_set_digits.argtypes = [vp, vp]
+        L.emu_res64.argtypes = [vp]
+        L.emu_res64.restype = ctypes.c_uint64
+        L.emu_levels.argtypes = [vp]
+        L.emu_levels.restype = u32
+        _emu = L
+    return _emu
+
+
+def golden_cases():
+    with open(GOLDEN) as f:
+        return json.load(f)["cases"]
+
+
+def golden(p, mode, iters=None):
+    """All GMP res64 checkpoints recorded for (p, mode), merged over fixture cases."""
+    out = {}
+    for c in golden_cases():
+        if c["p"] == p and c["mode"] == mode and (iters is None or c["iters"] == iters):
+            out.update({int(k): int(v, 16) for k, v in c["res64"].items()})
+    return out or None
+
+
+def widths(p, n):
+    e = [-(-p * i // n) for i in range(n + 1)]
+    return [e[i + 1] - e[i] for i in range(n)]
+
+
+def digits_of(value, p, n):
+    """Natural digits of an integer value (0 <= value < 2^p)."""
+    w = widths(p, n)
+    out = []
+    for wi in w:
+        out.append(value & ((1 << wi) - 1))
+        value >>= wi
+    return out
+
+
+def value_of(d, p, n):
+    w = widths(p, n)
+    v, e = 0, 0
+    for di, wi in zip(d, w):
+        v += di << e
+        e += wi
+    return v
+
+
+def ll_python(p, k, s=4, prp=False):
+    """s_k by Python integers (small p only)."""
+    M = (1 << p) - 1
+    for _ in range(k):
+        s = s * s if prp else s * s - 2
+        s %= M
+    return s

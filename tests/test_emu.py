@@ -1,0 +1,86 @@
+"""The HIP kernel bodies run on the CPU (tests/emu: OpenMP threads as workgroup lanes,
+bounds-checked LDS and global indexing) against the oracle, bit for bit."""
+import ctypes
+import os
+import random
+
+import pytest
+
+from helpers import OracleTest, emu, golden
+
+os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
+
+
+class Emu:
+    def __init__(self, p, n, prp=False, force_lc=0):
+        self.L = emu()
+        self.n = n
+        self.h = self.L.emu_create(p, n, int(prp), force_lc)
+        assert self.h, (p, n)
+
+    def __del__(self):
+        try:
+            self.L.emu_destroy(self.h)
+        except Exception:
+            pass
+
+    def iterate(self, k):
+        assert self.L.emu_iterate(self.h, k) == 0
+
+    def digits(self):
+        b = (ctypes.c_uint64 * self.n)()
+        self.L.emu_get_digits(self.h, b)
+        return list(b)
+
+    def set_digits(self, d):
+        self.L.emu_set_digits(self.h, (ctypes.c_uint64 * self.n)(*d))
+
+    def res64(self):
+        return int(self.L.emu_res64(self.h))
+
+
+CASES = [  # (p, n, force_lc): small path 8..4096, large path 2^13..2^17 with several splits
+    (131, 8, 0), (521, 32, 0), (1279, 64, 0), (2203, 128, 0), (4423, 256, 0), (9689, 512, 0),
+    (19937, 1024, 0), (44497, 2048, 0), (86243, 4096, 0),
+    (86243, 8192, 0), (200003, 16384, 0), (400009, 32768, 0), (400009, 32768, 9),
+    (1000003, 65536, 0), (1000003, 65536, 10), (2000003, 131072, 0),
+]
+
+
+@pytest.mark.parametrize("p,n,lc", CASES)
+@pytest.mark.parametrize("prp", [False, True])
+def test_emulated_kernels_match_oracle_random_state(p, n, lc, prp):
+    rnd = random.Random(p * 7 + n + prp)
+    o = OracleTest(p, n, prp)
+    w = o.widths()
+    d = [rnd.getrandbits(wi) for wi in w]
+    o.set_digits(d)
+    e = Emu(p, n, prp, lc)
+    e.set_digits(d)
+    k = 3 if n >= 32768 else 6
+    o.iterate(k)
+    e.iterate(k)
+    assert e.digits() == o.digits()
+
+
+@pytest.mark.parametrize("p,n", [(9689, 512), (86243, 4096), (86243, 8192)])
+def test_emulated_orbit_matches_gmp(p, n):
+    g = golden(p, "LL")
+    e = Emu(p, n)
+    done = 0
+    for k in sorted(g):
+        if k > 3000:
+            break
+        e.iterate(k - done)
+        done = k
+        assert e.res64() == g[k], k
+
+
+def test_emulated_full_small_mersenne_prime():
+    e = Emu(9689, 512)
+    e.iterate(9687)
+    assert e.res64() == 0 and set(e.digits()) == {0}
+    e = Emu(9689, 512, prp=True)
+    e.iterate(9689)
+    d = e.digits()
+    assert d[0] == 9 and set(d[1:]) == {0}

@@ -1,0 +1,141 @@
+"""GPU parity: the HIP engine (through the C ABI) against GMP fixtures and the oracle.
+
+Bar: bit-exact residues.  Sizes the oracle finishes in seconds are compared digit for
+digit; the full-size configurations are compared through GMP res64 checkpoints.
+"""
+import random
+import sys
+
+import pytest
+
+from helpers import OracleTest, digits_of, golden, golden_cases, value_of, widths
+
+import cuda_amd
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+SMALL_KAT = [c for c in golden_cases() if c["p"] <= 23209]
+
+
+@pytest.mark.parametrize("case", SMALL_KAT, ids=lambda c: f"{c['mode']}{c['p']}")
+def test_small_exponents_match_gmp(case):
+    p, prp = case["p"], case["mode"] == "PRP"
+    say("case", case["mode"], p)
+    t = cuda_amd.LucasPRP(p, prp=prp)
+    done = 0
+    for k, r in sorted((int(k), int(v, 16)) for k, v in case["res64"].items()):
+        t.iterate(k - done)
+        done = k
+        assert t.res64() == r, (p, case["mode"], k)
+    t.close()
+
+
+def test_m86243_full_ll_every_10000():
+    """Plain single LL test of M86243 at n = 4096: res64 at every 10,000 iterations and
+    residue 0 at the end."""
+    g = golden(86243, "LL")
+    t = cuda_amd.LucasPRP(86243)
+    assert t.n == 4096
+    done = 0
+    for k in list(range(10000, 86241, 10000)) + [86241]:
+        say("M86243 iteration", k)
+        t.iterate(k - done)
+        done = k
+        assert t.res64() == g[k], k
+    d = t.get_digits()
+    assert set(d) == {0}
+
+
+@pytest.mark.parametrize("p,n,lc", [
+    (86243, 8192, 0), (200003, 16384, 0), (400009, 32768, 0), (1000003, 65536, 0),
+    (2000003, 131072, 0), (4000037, 262144, 0), (8000009, 524288, 0), (16000057, 1 << 20, 0),
+    (33000011, 1 << 21, 0), (70000027, 1 << 22, 0),
+])
+@pytest.mark.parametrize("prp", [False, True])
+def test_large_random_state_matches_oracle(p, n, lc, prp):
+    say("large random", p, n, prp)
+    rnd = random.Random(p + n + prp)
+    o = OracleTest(p, n, prp, n0=1 if n <= 65536 else 1024)
+    w = o.widths()
+    d = [rnd.getrandbits(wi) for wi in w]
+    o.set_digits(d)
+    t = cuda_amd.LucasPRP(p, n, prp=prp)
+    t.set_digits(d)
+    assert t.get_digits() == d  # untouched residue reads back as given
+    k = 2 if n >= (1 << 20) else 4
+    o.iterate(k)
+    t.iterate(k)
+    assert t.get_digits() == o.digits()
+    t.close()
+
+
+def test_c2_res64_checkpoints():
+    """p = 70,000,027, n = 2^22 (the throughput configuration): res64 at every 1,000
+    iterations that GMP has been run to."""
+    g = golden(70000027, "LL")
+    if not g:
+        pytest.skip("no GMP fixture for p=70000027 yet")
+    t = cuda_amd.LucasPRP(70000027)
+    assert t.n == 1 << 22
+    done = 0
+    for k in sorted(g):
+        say("C2 iteration", k)
+        t.iterate(k - done)
+        done = k
+        assert t.res64() == g[k], k
+
+
+def test_c3_prp_res64_checkpoints():
+    g = golden(140000041, "PRP")
+    if not g:
+        pytest.skip("no GMP fixture for p=140000041 yet")
+    t = cuda_amd.LucasPRP(140000041, prp=True)
+    assert t.n == 1 << 23
+    done = 0
+    for k in sorted(g):
+        say("C3 iteration", k)
+        t.iterate(k - done)
+        done = k
+        assert t.res64() == g[k], k
+
+
+@pytest.mark.parametrize("p,n", [(521, 32), (86243, 4096), (400009, 32768)])
+def test_edge_values(p, n):
+    """0, 1, 2, M-1 and the all-ones form of zero, one iteration each (wrap-around carry
+    and borrow, and the reference's PRP all-ones convention)."""
+    M = (1 << p) - 1
+    w = widths(p, n)
+    for v in (0, 1, 2, 3, M - 1, M - 2):
+        for prp in (False, True):
+            t = cuda_amd.LucasPRP(p, n, prp=prp)
+            t.set_digits(digits_of(v, p, n))
+            t.iterate(1)
+            assert value_of(t.get_digits(), p, n) == (v * v - (0 if prp else 2)) % M, (v, prp)
+            t.close()
+    ones = [(1 << wi) - 1 for wi in w]
+    t = cuda_amd.LucasPRP(p, n)
+    t.set_digits(ones)
+    t.iterate(1)
+    assert value_of(t.get_digits(), p, n) == M - 2
+    t = cuda_amd.LucasPRP(p, n, prp=True)
+    t.set_digits(ones)
+    t.iterate(2)
+    assert t.get_digits() == ones and t.res64() == 0
+
+
+def test_tune_keeps_residue():
+    t = cuda_amd.LucasPRP(2000003, 1 << 17)
+    u = cuda_amd.LucasPRP(2000003, 1 << 17)
+    t.iterate(50)
+    u.iterate(50)
+    t.tune(20)
+    t.iterate(50)
+    u.iterate(50)
+    assert t.res64() == u.res64()
+    assert t.get_digits() == u.get_digits()
