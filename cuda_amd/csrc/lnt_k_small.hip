@@ -1,18 +1,28 @@
 // Instantiations of the single-workgroup kernel for n = 8 .. 4096.
 #include "lnt_launch.hpp"
 
-template <int B1, int B2>
-static bool go_small(const lnt::Plan &pl, const LntParams &P, u32 iters, bool sub2_first, bool ll, hipStream_t s) {
+template <int B1, int B2, int M>
+static bool go_small_m(const LntParams &P, u32 iters, bool sub2_first, bool ll, hipStream_t s, int mode) {
     dim3 grid(1), block(B2);
-    if (pl.M == 4)
-        hipLaunchKernelGGL((lnt_k_small<B1, B2, 4>), grid, block, 0, s, P, iters, (int)sub2_first, (int)ll);
+    if (mode == LNT_FORWARD)
+        hipLaunchKernelGGL((lnt_k_small<B1, B2, M, LNT_FORWARD>), grid, block, 0, s, P, 1u, (int)sub2_first, (int)ll);
+    else if (mode == LNT_MULTIPLY)
+        hipLaunchKernelGGL((lnt_k_small<B1, B2, M, LNT_MULTIPLY>), grid, block, 0, s, P, 1u, (int)sub2_first, (int)ll);
     else
-        hipLaunchKernelGGL((lnt_k_small<B1, B2, 8>), grid, block, 0, s, P, iters, (int)sub2_first, (int)ll);
+        hipLaunchKernelGGL((lnt_k_small<B1, B2, M, LNT_SQUARE>), grid, block, 0, s, P, iters, (int)sub2_first, (int)ll);
     return true;
 }
 
-bool lnt_launch_small(const lnt::Plan &pl, const LntParams &P, u32 iters, bool sub2_first, bool ll, hipStream_t s) {
-#define SM(b1, b2) if (pl.row.B1 == b1 && pl.row.B2 == b2) return go_small<b1, b2>(pl, P, iters, sub2_first, ll, s);
+template <int B1, int B2>
+static bool go_small(const lnt::Plan &pl, const LntParams &P, u32 iters, bool sub2_first, bool ll, hipStream_t s,
+                     int mode) {
+    return pl.M == 4 ? go_small_m<B1, B2, 4>(P, iters, sub2_first, ll, s, mode)
+                     : go_small_m<B1, B2, 8>(P, iters, sub2_first, ll, s, mode);
+}
+
+bool lnt_launch_small(const lnt::Plan &pl, const LntParams &P, u32 iters, bool sub2_first, bool ll, hipStream_t s,
+                      int mode) {
+#define SM(b1, b2) if (pl.row.B1 == b1 && pl.row.B2 == b2) return go_small<b1, b2>(pl, P, iters, sub2_first, ll, s, mode);
     SM(4, 2) SM(4, 4) SM(8, 4) SM(8, 8) SM(16, 8) SM(16, 16) SM(32, 16) SM(32, 32) SM(64, 32) SM(64, 64)
 #undef SM
     return false;

@@ -46,6 +46,7 @@ struct LntParams {
     GView<const u64> tw4, itw4;  // 4-step twiddles (large path), natural slots
     GView<const u64> twr, itwr;  // row-line twiddles (C words)
     GView<const u64> twc, itwc;  // column-line twiddles (R words)
+    GView<const u64> f;      // n words: the other operand of a product, transform domain
     u32 p, n, log_n, C, R;
     u32 levels;              // carry levels m
 };
@@ -199,7 +200,11 @@ LNT_KFN void lnt_col_inv_body(int tid, int bid, LView lds, const LntParams &P) {
 // ------------------------------------------------------------------ row pass
 // grid: R / G workgroups, block: G * B2 threads (G rows per workgroup).
 // LDS words: G * B1 * (B2 + PAD).
-template <int B1, int B2, int G>
+// MODE 0: square (one iteration).  MODE 1: forward only; the transform-domain words go
+// to slot base + t*B1 + i of y.  MODE 2: forward, multiply by the other operand's words f
+// (same slots, written by a MODE 1 pass), inverse.
+constexpr int LNT_SQUARE = 0, LNT_FORWARD = 1, LNT_MULTIPLY = 2;
+template <int B1, int B2, int G, int MODE = LNT_SQUARE>
 LNT_KFN void lnt_row_body(int tid, int bid, LView lds, const LntParams &P) {
     constexpr int RP = B1 / B2;
     constexpr int S = B2 + LNT_PAD;
@@ -224,12 +229,24 @@ LNT_KFN void lnt_row_body(int tid, int bid, LView lds, const LntParams &P) {
 #pragma unroll
         for (int i1 = 0; i1 < B2; i1++) a[q * B2 + i1] = lds[lb + (u32)(k2 * S + i1)];
         dft_fwd<B2>(a + q * B2);
+        if (MODE == LNT_FORWARD) {
 #pragma unroll
-        for (int i1 = 0; i1 < B2; i1++) a[q * B2 + i1] = gl_sqr(a[q * B2 + i1]);
+            for (int i1 = 0; i1 < B2; i1++) P.y[base + (u64)t * B1 + q * B2 + i1] = a[q * B2 + i1];
+            continue;
+        }
+        if (MODE == LNT_MULTIPLY) {
+#pragma unroll
+            for (int i1 = 0; i1 < B2; i1++)
+                a[q * B2 + i1] = gl_mul(a[q * B2 + i1], P.f[base + (u64)t * B1 + q * B2 + i1]);
+        } else {
+#pragma unroll
+            for (int i1 = 0; i1 < B2; i1++) a[q * B2 + i1] = gl_sqr(a[q * B2 + i1]);
+        }
         dft_inv<B2>(a + q * B2);
 #pragma unroll
         for (int i1 = 0; i1 < B2; i1++) lds[lb + (u32)(k2 * S + i1)] = a[q * B2 + i1];
     }
+    if (MODE == LNT_FORWARD) return;
     LNT_SYNC();
 #pragma unroll
     for (int rho = 0; rho < B1; rho++) {
@@ -244,7 +261,9 @@ LNT_KFN void lnt_row_body(int tid, int bid, LView lds, const LntParams &P) {
 // ------------------------------------------------------------------ small residues
 // One workgroup of B2 threads runs `iters` whole iterations on n = B1 * B2 words kept
 // in LDS (natural order, field elements).  LDS words: max(n, B1 * (B2 + PAD)).
-template <int B1, int B2, int M>
+// MODE 0: `iters` squarings.  MODE 1 (iters = 1): forward transform of x into y (slot
+// t*B1 + i), x unchanged.  MODE 2 (iters = 1): x <- x * (the operand whose transform is f).
+template <int B1, int B2, int M, int MODE = LNT_SQUARE>
 LNT_KFN void lnt_small_body(int tid, LView lds, const LntParams &P, u32 iters, bool sub2_first,
                             bool ll) {
     constexpr int RP = B1 / B2;
@@ -307,12 +326,24 @@ LNT_KFN void lnt_small_body(int tid, LView lds, const LntParams &P, u32 iters, b
 #pragma unroll
             for (int i1 = 0; i1 < B2; i1++) a[q * B2 + i1] = lds[(u32)(k2 * S + i1)];
             dft_fwd<B2>(a + q * B2);
+            if (MODE == LNT_FORWARD) {
 #pragma unroll
-            for (int i1 = 0; i1 < B2; i1++) a[q * B2 + i1] = gl_sqr(a[q * B2 + i1]);
+                for (int i1 = 0; i1 < B2; i1++) P.y[(u64)t * B1 + q * B2 + i1] = a[q * B2 + i1];
+                continue;
+            }
+            if (MODE == LNT_MULTIPLY) {
+#pragma unroll
+                for (int i1 = 0; i1 < B2; i1++)
+                    a[q * B2 + i1] = gl_mul(a[q * B2 + i1], P.f[(u64)t * B1 + q * B2 + i1]);
+            } else {
+#pragma unroll
+                for (int i1 = 0; i1 < B2; i1++) a[q * B2 + i1] = gl_sqr(a[q * B2 + i1]);
+            }
             dft_inv<B2>(a + q * B2);
 #pragma unroll
             for (int i1 = 0; i1 < B2; i1++) lds[(u32)(k2 * S + i1)] = a[q * B2 + i1];
         }
+        if (MODE == LNT_FORWARD) return;
         LNT_SYNC();
 #pragma unroll
         for (int rho = 0; rho < B1; rho++) {

@@ -8,11 +8,11 @@
 #include "lnt_host.hpp"
 #include "lnt_kernels.hpp"
 
-template <int B1, int B2, int M>
+template <int B1, int B2, int M, int MODE>
 __global__ __launch_bounds__(B2 < 64 ? 64 : B2) void lnt_k_small(LntParams P, u32 iters, int sub2_first, int ll) {
     constexpr u32 W = (B1 * B2 > B1 * (B2 + LNT_PAD)) ? B1 * B2 : B1 * (B2 + LNT_PAD);
     __shared__ u64 smem[W];
-    lnt_small_body<B1, B2, M>(threadIdx.x, LView{smem, W}, P, iters, sub2_first != 0, ll != 0);
+    lnt_small_body<B1, B2, M, MODE>(threadIdx.x, LView{smem, W}, P, iters, sub2_first != 0, ll != 0);
 }
 
 template <int B1, int B2, int K, int M>
@@ -30,16 +30,17 @@ __global__ __launch_bounds__(K * B2) void lnt_k_col_inv(LntParams P) {
     lnt_col_inv_body<B1, B2, K>(threadIdx.x, blockIdx.x, LView{smem, W}, P);
 }
 
-template <int B1, int B2, int G>
+template <int B1, int B2, int G, int MODE>
 __global__ __launch_bounds__(G * B2) void lnt_k_row(LntParams P) {
     constexpr u32 W = G * B1 * (B2 + LNT_PAD);
     __shared__ u64 smem[W];
-    lnt_row_body<B1, B2, G>(threadIdx.x, blockIdx.x, LView{smem, W}, P);
+    lnt_row_body<B1, B2, G, MODE>(threadIdx.x, blockIdx.x, LView{smem, W}, P);
 }
 
 // Launchers (defined in lnt_k_small.hip / lnt_k_col.hip / lnt_k_row.hip).
+// mode: LNT_SQUARE (iters squarings), LNT_FORWARD / LNT_MULTIPLY (one pass, iters = 1)
 bool lnt_launch_small(const lnt::Plan &pl, const LntParams &P, u32 iters, bool sub2_first, bool ll,
-                      hipStream_t s);
+                      hipStream_t s, int mode = LNT_SQUARE);
 bool lnt_launch_col_fwd(const lnt::Plan &pl, const LntParams &P, bool sub2, hipStream_t s);
 bool lnt_launch_col_inv(const lnt::Plan &pl, const LntParams &P, hipStream_t s);
-bool lnt_launch_row(const lnt::Plan &pl, const LntParams &P, hipStream_t s);
+bool lnt_launch_row(const lnt::Plan &pl, const LntParams &P, hipStream_t s, int mode = LNT_SQUARE);
