@@ -2,6 +2,7 @@
 // launcher, readout, timing and the validating tuner.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -57,8 +58,21 @@ struct lnt_ctx {
     std::vector<uint8_t> wid;
     u64 *x = nullptr, *y = nullptr, *w = nullptr, *uw = nullptr;
     u64 *tw4 = nullptr, *itw4 = nullptr, *twr = nullptr, *itwr = nullptr, *twc = nullptr, *itwc = nullptr;
+    u64 *f = nullptr;  // transform of the second operand of a product (allocated on first use)
     LntParams P{};
     std::string err;
+    // Gerbicz-checked PRP (lnt_gerbicz_*): residue slots on the device
+    struct Slot {
+        u64 *x = nullptr;
+        bool pending = false;
+    };
+    struct Gerbicz {
+        bool on = false;
+        uint32_t B = 0, L = 0;
+        uint64_t it = 0, it_v = 0, checks = 0, rollbacks = 0, fails_here = 0;
+        uint64_t ck_res64 = 0;
+        Slot U0, D, Dprev, E, Uv, Dv;
+    } g;
 };
 
 static std::mutex g_err_mu;
@@ -83,7 +97,9 @@ static int fail(lnt_ctx *c, int code, const std::string &msg) {
     } while (0)
 
 static void free_buffers(lnt_ctx *c) {
-    u64 **bufs[] = {&c->x, &c->y, &c->w, &c->uw, &c->tw4, &c->itw4, &c->twr, &c->itwr, &c->twc, &c->itwc};
+    u64 **bufs[] = {&c->x,      &c->y,      &c->w,      &c->uw,      &c->tw4,     &c->itw4,   &c->twr,
+                    &c->itwr,   &c->twc,    &c->itwc,   &c->f,       &c->g.U0.x,  &c->g.D.x,  &c->g.Dprev.x,
+                    &c->g.E.x,  &c->g.Uv.x, &c->g.Dv.x};
     for (u64 **b : bufs)
         if (*b) {
             (void)hipFree(*b);
@@ -107,8 +123,8 @@ static int build_tables(lnt_ctx *c) {
     const u32 Lr = pl.small ? n : pl.C;
     HIPCHK(c, hipMalloc(&c->twr, (size_t)Lr * 8), "hipMalloc (row twiddles)");
     HIPCHK(c, hipMalloc(&c->itwr, (size_t)Lr * 8), "hipMalloc (row twiddles)");
+    HIPCHK(c, hipMalloc(&c->y, nb), "hipMalloc (work buffer)");
     if (!pl.small) {
-        HIPCHK(c, hipMalloc(&c->y, nb), "hipMalloc (work buffer)");
         HIPCHK(c, hipMalloc(&c->tw4, nb), "hipMalloc (4-step twiddles)");
         HIPCHK(c, hipMalloc(&c->itw4, nb), "hipMalloc (4-step twiddles)");
         HIPCHK(c, hipMalloc(&c->twc, (size_t)pl.R * 8), "hipMalloc (column twiddles)");
@@ -138,7 +154,7 @@ static int build_tables(lnt_ctx *c) {
     HIPCHK(c, hipGetLastError(), "table kernels launch");
     LntParams &P = c->P;
     P.x = view<u64>(c->x, n);
-    P.y = view<u64>(c->y, pl.small ? 0 : n);
+    P.y = view<u64>(c->y, n);
     P.w = view<const u64>(c->w, n);
     P.uw = view<const u64>(c->uw, n);
     P.tw4 = view<const u64>(c->tw4, pl.small ? 0 : n);
@@ -175,30 +191,85 @@ static int upload_digits(lnt_ctx *c, const uint64_t *d) {
     return LNT_OK;
 }
 
-// One or more iterations; large path: three launches each.
-static int run_iterations(lnt_ctx *c, uint32_t k) {
+// Parameters with the residue buffer x, work buffer y and product operand f swapped in.
+static LntParams params_for(const lnt_ctx *c, u64 *x, u64 *y, const u64 *f) {
+    LntParams P = c->P;
+    P.x = view<u64>(x, c->pl.n);
+    P.y = view<u64>(y, c->pl.n);
+    P.f = view<const u64>(f, f ? c->pl.n : 0);
+    return P;
+}
+
+// k squarings (and -2 for LL) of the residue in buffer x; large path: three launches each.
+static int square_buf(lnt_ctx *c, u64 *x, bool &pending, uint32_t k) {
     const Plan &pl = c->pl;
     const bool ll = !c->prp;
+    const LntParams P = params_for(c, x, c->y, nullptr);
     if (pl.small) {
         // bounded launches keep each kernel short
         const uint32_t chunk = pl.n >= 2048 ? 256 : 1024;
         while (k) {
             uint32_t s = k < chunk ? k : chunk;
-            if (!lnt_launch_small(pl, c->P, s, ll && c->pending, ll, c->stream))
+            if (!lnt_launch_small(pl, P, s, ll && pending, ll, c->stream))
                 return fail(c, LNT_UNSUPPORTED, "no small-path kernel for this length");
-            c->pending = true;
+            pending = true;
             k -= s;
         }
     } else {
         for (uint32_t i = 0; i < k; i++) {
-            if (!lnt_launch_col_fwd(pl, c->P, ll && c->pending, c->stream) || !lnt_launch_row(pl, c->P, c->stream) ||
-                !lnt_launch_col_inv(pl, c->P, c->stream))
+            if (!lnt_launch_col_fwd(pl, P, ll && pending, c->stream) || !lnt_launch_row(pl, P, c->stream) ||
+                !lnt_launch_col_inv(pl, P, c->stream))
                 return fail(c, LNT_UNSUPPORTED, "no kernel for this row/column split");
-            c->pending = true;
+            pending = true;
         }
     }
     HIPCHK(c, hipGetLastError(), "kernel launch");
-    c->pristine = false;
+    return LNT_OK;
+}
+
+static int run_iterations(lnt_ctx *c, uint32_t k) {
+    int rc = square_buf(c, c->x, c->pending, k);
+    if (rc == LNT_OK) c->pristine = false;
+    return rc;
+}
+
+// dst <- dst * src mod M_p (both buffers hold residues of this context's shape).
+// The transform of src goes to f, dst is transformed, multiplied and inverted in y.
+static int mul_buf(lnt_ctx *c, u64 *dst, bool &dst_pending, const u64 *src, bool src_pending) {
+    const Plan &pl = c->pl;
+    const bool ll = !c->prp;
+    if (!c->f) HIPCHK(c, hipMalloc(&c->f, (size_t)pl.n * 8), "hipMalloc (product operand)");
+    const LntParams Ps = params_for(c, const_cast<u64 *>(src), c->f, nullptr);
+    const LntParams Pd = params_for(c, dst, c->y, c->f);
+    bool ok;
+    if (pl.small) {
+        ok = lnt_launch_small(pl, Ps, 1, ll && src_pending, ll, c->stream, LNT_FORWARD) &&
+             lnt_launch_small(pl, Pd, 1, ll && dst_pending, ll, c->stream, LNT_MULTIPLY);
+    } else {
+        ok = lnt_launch_col_fwd(pl, Ps, ll && src_pending, c->stream) &&
+             lnt_launch_row(pl, Ps, c->stream, LNT_FORWARD) &&
+             lnt_launch_col_fwd(pl, Pd, ll && dst_pending, c->stream) &&
+             lnt_launch_row(pl, Pd, c->stream, LNT_MULTIPLY) && lnt_launch_col_inv(pl, Pd, c->stream);
+    }
+    if (!ok) return fail(c, LNT_UNSUPPORTED, "no kernel for this length");
+    HIPCHK(c, hipGetLastError(), "kernel launch");
+    dst_pending = false;  // a product owes no "-2"
+    return LNT_OK;
+}
+
+static int copy_buf(lnt_ctx *c, u64 *dst, const u64 *src) {
+    HIPCHK(c, hipMemcpyAsync(dst, src, (size_t)c->pl.n * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpy (copy)");
+    return LNT_OK;
+}
+
+// Canonical digits of the residue in a device buffer (waits for the stream).
+static int canonical_of(lnt_ctx *c, const u64 *buf, bool pending, std::vector<uint64_t> &d) {
+    const u32 n = c->pl.n;
+    std::vector<uint64_t> xd(n);
+    d.resize(n);
+    HIPCHK(c, hipMemcpyAsync(xd.data(), buf, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpy (D2H)");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    to_canonical_digits(c->wid, xd.data(), !c->prp && pending, d.data());
     return LNT_OK;
 }
 
@@ -484,6 +555,157 @@ int lnt_tune(lnt_ctx *c, int loops) {
     std::swap(c->itwc, nc->itwc);
     std::swap(c->P, nc->P);
     lnt_destroy(nc);
+    return LNT_OK;
+}
+
+// ------------------------------------------------------------------ products, copies, faults
+
+static bool same_shape(const lnt_ctx *a, const lnt_ctx *b) {
+    return a->device == b->device && a->p == b->p && a->prp == b->prp && a->pl.n == b->pl.n && a->pl.C == b->pl.C &&
+           a->pl.small == b->pl.small;
+}
+
+int lnt_mul(lnt_ctx *dst, lnt_ctx *src) {
+    if (!dst || !src) return fail(dst, LNT_INVALID_ARGUMENT, "NULL context");
+    if (!same_shape(dst, src)) return fail(dst, LNT_INVALID_ARGUMENT, "lnt_mul: contexts differ in p, n, mode or split");
+    HIPCHK(dst, hipSetDevice(dst->device), "hipSetDevice");
+    if (src != dst) HIPCHK(dst, hipStreamSynchronize(src->stream), "hipStreamSynchronize");
+    int rc = mul_buf(dst, dst->x, dst->pending, src->x, src->pending);
+    if (rc == LNT_OK) dst->pristine = false;
+    return rc;
+}
+
+int lnt_copy(lnt_ctx *dst, lnt_ctx *src) {
+    if (!dst || !src) return fail(dst, LNT_INVALID_ARGUMENT, "NULL context");
+    if (!same_shape(dst, src)) return fail(dst, LNT_INVALID_ARGUMENT, "lnt_copy: contexts differ in p, n, mode or split");
+    HIPCHK(dst, hipSetDevice(dst->device), "hipSetDevice");
+    if (src != dst) HIPCHK(dst, hipStreamSynchronize(src->stream), "hipStreamSynchronize");
+    int rc = copy_buf(dst, dst->x, src->x);
+    if (rc) return rc;
+    dst->pending = src->pending;
+    dst->pristine = src->pristine;
+    dst->zero_ones = src->zero_ones;
+    dst->set_digits = src->set_digits;
+    return LNT_OK;
+}
+
+int lnt_inject_error(lnt_ctx *c, uint32_t word, uint64_t xor_mask) {
+    if (!c) return fail(c, LNT_INVALID_ARGUMENT, "NULL context");
+    if (word >= c->pl.n) return fail(c, LNT_INVALID_ARGUMENT, "lnt_inject_error: word index out of range");
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    uint64_t v = 0;
+    HIPCHK(c, hipMemcpyAsync(&v, c->x + word, 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpy (D2H)");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    // flip bits of the word's signed value, keeping it a valid field encoding
+    int64_t sv = gl_to_i64(v);
+    sv ^= (int64_t)(xor_mask & 0x3FFFFFFFFFFFull);
+    v = gl_from_i64(sv);
+    HIPCHK(c, hipMemcpyAsync(c->x + word, &v, 8, hipMemcpyHostToDevice, c->stream), "hipMemcpy (H2D)");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    c->pristine = false;
+    return LNT_OK;
+}
+
+// ------------------------------------------------------------------ Gerbicz-checked PRP
+// Residues u_k = 3^(2^k) (k squarings since lnt_gerbicz_begin, u_0 = the residue then).
+// Every B iterations D <- D * u (D = prod u_(iB), i = 0..m); every L blocks the check
+// D_m == u_0 * D_(m-1)^(2^B) (B extra squarings of a copy of D_(m-1)).  A pass saves
+// (u, D, k) as the verified checkpoint on the device; a failure restores it and redoes the
+// iterations since.
+
+static int gz_alloc(lnt_ctx *c, lnt_ctx::Slot &s) {
+    if (!s.x) HIPCHK(c, hipMalloc(&s.x, (size_t)c->pl.n * 8), "hipMalloc (Gerbicz slot)");
+    return LNT_OK;
+}
+
+static int gz_check(lnt_ctx *c, bool &passed) {
+    auto &g = c->g;
+    int rc;
+    if ((rc = copy_buf(c, g.E.x, g.Dprev.x))) return rc;
+    g.E.pending = g.Dprev.pending;
+    if ((rc = square_buf(c, g.E.x, g.E.pending, g.B))) return rc;
+    if ((rc = mul_buf(c, g.E.x, g.E.pending, g.U0.x, g.U0.pending))) return rc;
+    std::vector<uint64_t> de, dd;
+    if ((rc = canonical_of(c, g.E.x, g.E.pending, de))) return rc;
+    if ((rc = canonical_of(c, g.D.x, g.D.pending, dd))) return rc;
+    passed = de == dd;
+    if (passed) g.ck_res64 = res64(c->wid, dd.data());
+    return LNT_OK;
+}
+
+int lnt_gerbicz_begin(lnt_ctx *c, uint32_t block, uint32_t blocks_per_check) {
+    if (!c) return fail(c, LNT_INVALID_ARGUMENT, "NULL context");
+    if (!c->prp) return fail(c, LNT_INVALID_ARGUMENT, "the Gerbicz check applies to PRP tests");
+    if (block == 0 || blocks_per_check == 0) return fail(c, LNT_INVALID_ARGUMENT, "block sizes must be positive");
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    auto &g = c->g;
+    int rc;
+    for (auto *s : {&g.U0, &g.D, &g.Dprev, &g.E, &g.Uv, &g.Dv})
+        if ((rc = gz_alloc(c, *s))) return rc;
+    g.on = true;
+    g.B = block;
+    g.L = blocks_per_check;
+    g.it = g.it_v = g.checks = g.rollbacks = g.fails_here = 0;
+    for (auto *s : {&g.U0, &g.D, &g.Dprev, &g.Uv, &g.Dv}) {
+        if ((rc = copy_buf(c, s->x, c->x))) return rc;
+        s->pending = c->pending;
+    }
+    std::vector<uint64_t> d;
+    if ((rc = canonical_of(c, g.D.x, g.D.pending, d))) return rc;
+    g.ck_res64 = res64(c->wid, d.data());
+    return LNT_OK;
+}
+
+int lnt_gerbicz_iterate(lnt_ctx *c, uint32_t k) {
+    if (!c) return fail(c, LNT_INVALID_ARGUMENT, "NULL context");
+    auto &g = c->g;
+    if (!g.on) return fail(c, LNT_INVALID_ARGUMENT, "lnt_gerbicz_begin was not called");
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    const uint64_t target = g.it + k;
+    int rc;
+    while (g.it < target) {
+        uint64_t step = std::min<uint64_t>(target - g.it, g.B - g.it % g.B);
+        if ((rc = square_buf(c, c->x, c->pending, (uint32_t)step))) return rc;
+        c->pristine = false;
+        g.it += step;
+        if (g.it % g.B) continue;
+        // block boundary: D_prev <- D, D <- D * u
+        if ((rc = copy_buf(c, g.Dprev.x, g.D.x))) return rc;
+        g.Dprev.pending = g.D.pending;
+        if ((rc = mul_buf(c, g.D.x, g.D.pending, c->x, c->pending))) return rc;
+        if ((g.it / g.B) % g.L) continue;
+        bool passed = false;
+        if ((rc = gz_check(c, passed))) return rc;
+        if (passed) {
+            g.checks++;
+            g.fails_here = 0;
+            g.it_v = g.it;
+            if ((rc = copy_buf(c, g.Uv.x, c->x))) return rc;
+            g.Uv.pending = c->pending;
+            if ((rc = copy_buf(c, g.Dv.x, g.D.x))) return rc;
+            g.Dv.pending = g.D.pending;
+        } else {
+            g.rollbacks++;
+            if (++g.fails_here > 3)
+                return fail(c, LNT_DEVICE_ERROR, "Gerbicz check failed 4 times from the same checkpoint");
+            if ((rc = copy_buf(c, c->x, g.Uv.x))) return rc;
+            c->pending = g.Uv.pending;
+            if ((rc = copy_buf(c, g.D.x, g.Dv.x))) return rc;
+            g.D.pending = g.Dv.pending;
+            g.it = g.it_v;
+        }
+    }
+    return LNT_OK;
+}
+
+int lnt_gerbicz_stats(lnt_ctx *c, uint64_t *out, int cap) {
+    if (!c || !out || cap < 5) return fail(c, LNT_INVALID_ARGUMENT, "bad argument");
+    const auto &g = c->g;
+    out[0] = g.it;
+    out[1] = g.it_v;
+    out[2] = g.checks;
+    out[3] = g.rollbacks;
+    out[4] = g.ck_res64;
     return LNT_OK;
 }
 

@@ -30,7 +30,8 @@ MODE_PRP = 1
 EXPORTS = (
     "lnt_create", "lnt_set_digits", "lnt_iterate", "lnt_get_digits", "lnt_res64",
     "lnt_sync", "lnt_tune", "lnt_destroy", "lnt_last_error", "lnt_secured_log2_length",
-    "lnt_length", "lnt_describe", "lnt_profile",
+    "lnt_length", "lnt_describe", "lnt_profile", "lnt_mul", "lnt_copy", "lnt_inject_error",
+    "lnt_gerbicz_begin", "lnt_gerbicz_iterate", "lnt_gerbicz_stats",
 )
 
 
@@ -74,6 +75,12 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lnt_describe.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
     lib.lnt_profile.argtypes = [vp, u32, ctypes.c_char_p, ctypes.c_size_t,
                                 ctypes.POINTER(ctypes.c_float), ctypes.c_int]
+    lib.lnt_mul.argtypes = [vp, vp]
+    lib.lnt_copy.argtypes = [vp, vp]
+    lib.lnt_inject_error.argtypes = [vp, u32, ctypes.c_uint64]
+    lib.lnt_gerbicz_begin.argtypes = [vp, u32, u32]
+    lib.lnt_gerbicz_iterate.argtypes = [vp, u32]
+    lib.lnt_gerbicz_stats.argtypes = [vp, u64p, ctypes.c_int]
     _lib = lib
     return lib
 
@@ -181,6 +188,35 @@ class LucasPRP:
 
     def tune(self, loops: int = 50) -> None:
         self._check(self._lib.lnt_tune(self._h, int(loops)))
+
+    # -- products, copies, faults, Gerbicz ------------------------------------------
+    def mul(self, other: "LucasPRP") -> None:
+        """self <- self * other mod M_p (same p, length, mode and split)."""
+        self._check(self._lib.lnt_mul(self._h, other._h))
+
+    def copy_from(self, other: "LucasPRP") -> None:
+        self._check(self._lib.lnt_copy(self._h, other._h))
+        self.iteration = other.iteration
+
+    def inject_error(self, word: int, xor_mask: int) -> None:
+        """Flip bits of one residue word on the device (a single-word corruption)."""
+        self._check(self._lib.lnt_inject_error(self._h, int(word), int(xor_mask)))
+
+    def gerbicz_begin(self, block: int = 1000, blocks_per_check: int = 10) -> None:
+        """Start the Gerbicz-checked PRP from the current residue (PRP tests only)."""
+        self._check(self._lib.lnt_gerbicz_begin(self._h, int(block), int(blocks_per_check)))
+
+    def gerbicz_iterate(self, k: int) -> None:
+        """k checked iterations; failed checks roll back to the last verified checkpoint and
+        redo the iterations since."""
+        self._check(self._lib.lnt_gerbicz_iterate(self._h, int(k)))
+        self.iteration += int(k)
+
+    def gerbicz_stats(self) -> dict:
+        out = (ctypes.c_uint64 * 5)()
+        self._check(self._lib.lnt_gerbicz_stats(self._h, out, 5))
+        return {"iteration": int(out[0]), "verified_iteration": int(out[1]), "checks": int(out[2]),
+                "rollbacks": int(out[3]), "checkpoint_res64": int(out[4])}
 
     def describe(self) -> List[str]:
         buf = ctypes.create_string_buffer(1024)

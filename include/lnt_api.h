@@ -2,8 +2,8 @@
  * liblnt -- C ABI of the MI355X (gfx950) Lucas-Lehmer / PRP squaring engine.
  *
  * Plain C types only: pointers, sizes, integers.  No exceptions cross this boundary;
- * every call returns an lnt_status and lnt_last_error() gives the message.  Host C++
- * (cuda_amd/host) and Python (cuda_amd/lnt.py, ctypes) bind these entry points.
+ * every call returns an lnt_status and lnt_last_error() gives the message.  The C++
+ * command line (cuda_amd/cli) and Python (cuda_amd/lnt.py, ctypes) bind these entry points.
  *
  * The reference drives its hot path through C++ classes on one CUDA stream
  * (LucasPRP, KernelPool, Kernel: lucas_prp.cuh:15-39, kernel.cuh:15-73); each entry
@@ -89,6 +89,29 @@ int lnt_describe(const lnt_ctx *ctx, char *buf, size_t buflen);
 /* Per-kernel average GPU durations (ms) over `iters` iterations, each kernel bracketed by
  * events on the context's stream.  names/ms receive up to `cap` entries; returns count. */
 int lnt_profile(lnt_ctx *ctx, uint32_t iters, char *names, size_t names_len, float *ms, int cap);
+
+/* ---- products, copies and the Gerbicz-checked PRP (new: the reference has no error
+ * check and no general multiply) ---- */
+
+/* dst <- dst * src mod M_p.  Both contexts must share p, n, mode and row/column split. */
+int lnt_mul(lnt_ctx *dst, lnt_ctx *src);
+
+/* dst <- src (residue and its bookkeeping). */
+int lnt_copy(lnt_ctx *dst, lnt_ctx *src);
+
+/* Test hook: flip bits (xor_mask, low 46 bits) of the signed value held in residue word
+ * `word` on the device, i.e. a single-word memory corruption. */
+int lnt_inject_error(lnt_ctx *ctx, uint32_t word, uint64_t xor_mask);
+
+/* PRP with the Gerbicz check: from the current residue u_0, every `block` iterations
+ * D <- D * u; every `blocks_per_check` blocks verify D_m == u_0 * D_(m-1)^(2^block), then
+ * save (u, D) as the verified checkpoint on the device, or restore the last verified
+ * checkpoint and redo the iterations since. */
+int lnt_gerbicz_begin(lnt_ctx *ctx, uint32_t block, uint32_t blocks_per_check);
+int lnt_gerbicz_iterate(lnt_ctx *ctx, uint32_t k);
+/* out[0] iterations since begin, [1] last verified iteration, [2] checks passed,
+ * [3] rollbacks, [4] res64 of D at the last verified checkpoint.  cap >= 5. */
+int lnt_gerbicz_stats(lnt_ctx *ctx, uint64_t *out, int cap);
 
 #ifdef __cplusplus
 }

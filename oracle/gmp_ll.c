@@ -11,10 +11,12 @@
  * used here are declared by hand (GMP manual, "Integer Functions"); mpz_t is the
  * documented {int alloc; int size; limb*} struct.  Link against libgmp.so.10.
  *
- * usage: gmp_ll <p> <LL|PRP> <iterations> <every> [dump_at]
+ * usage: gmp_ll <p> <LL|PRP> <iterations> <every> [dump_at] [block]
  *   prints "k res64hex" every <every> iterations and at <iterations>;
- *   with dump_at, writes the full residue at that k as little-endian u64 words
- *   to stdout-named file "gmp_p<p>_k<k>.bin".
+ *   with dump_at (0 = none), writes the full residue at that k as little-endian u64
+ *   words to the file "gmp_p<p>_k<k>.bin";
+ *   with block B, also keeps the Gerbicz product D = s_0 * s_B * s_2B * ... (mod M_p) and
+ *   prints "k res64hex dres64hex" (D after folding in s_k when B divides k).
  */
 #include <stdint.h>
 #include <stdio.h>
@@ -28,6 +30,7 @@ typedef mpz_struct mpz_t[1];
 extern void __gmpz_init(mpz_struct *);
 extern void __gmpz_clear(mpz_struct *);
 extern void __gmpz_set_ui(mpz_struct *, unsigned long);
+extern void __gmpz_set(mpz_struct *, const mpz_struct *);
 extern void __gmpz_mul(mpz_struct *, const mpz_struct *, const mpz_struct *);
 extern void __gmpz_add(mpz_struct *, const mpz_struct *, const mpz_struct *);
 extern void __gmpz_sub(mpz_struct *, const mpz_struct *, const mpz_struct *);
@@ -59,11 +62,13 @@ int main(int argc, char **argv) {
     unsigned long iters = strtoul(argv[3], NULL, 10);
     unsigned long every = strtoul(argv[4], NULL, 10);
     unsigned long dump_at = argc > 5 ? strtoul(argv[5], NULL, 10) : 0;
-    mpz_t s, t, M, two;
-    __gmpz_init(s); __gmpz_init(t); __gmpz_init(M); __gmpz_init(two);
+    unsigned long block = argc > 6 ? strtoul(argv[6], NULL, 10) : 0;
+    mpz_t s, t, M, two, D;
+    __gmpz_init(s); __gmpz_init(t); __gmpz_init(M); __gmpz_init(two); __gmpz_init(D);
     __gmpz_set_ui(M, 0); __gmpz_setbit(M, p); __gmpz_sub_ui(M, M, 1);
     __gmpz_set_ui(s, prp ? 3 : 4);
     __gmpz_set_ui(two, 2);
+    __gmpz_set(D, s);
     time_t t0 = time(NULL);
     for (unsigned long k = 1; k <= iters; k++) {
         __gmpz_mul(s, s, s);
@@ -72,9 +77,16 @@ int main(int argc, char **argv) {
             if (__gmpz_cmp(s, two) >= 0) __gmpz_sub_ui(s, s, 2);
             else { __gmpz_add(s, s, M); __gmpz_sub_ui(s, s, 2); }
         }
+        if (block && k % block == 0) {
+            __gmpz_mul(D, D, s);
+            mod_mersenne(D, t, M, p);
+        }
         if ((every && k % every == 0) || k == iters) {
             unsigned long lo = s->size == 0 ? 0 : s->d[0];
-            printf("%lu %016lX\n", k, lo);
+            if (block)
+                printf("%lu %016lX %016lX\n", k, lo, D->size == 0 ? 0UL : D->d[0]);
+            else
+                printf("%lu %016lX\n", k, lo);
             fflush(stdout);
         }
         if (dump_at && k == dump_at) {

@@ -21,7 +21,7 @@ struct Emu {
     Plan pl;
     uint32_t p;
     bool prp, pending;
-    std::vector<uint64_t> x, y, w, uw, tw4, itw4, twr, itwr, twc, itwc;
+    std::vector<uint64_t> x, y, f, w, uw, tw4, itw4, twr, itwr, twc, itwc;
     std::vector<uint8_t> wid;
     LntParams P;
 };
@@ -54,6 +54,7 @@ extern "C" void *emu_create(uint32_t p, uint32_t n, int prp, uint32_t force_lc) 
     const Plan &pl = e->pl;
     e->x.assign(n, 0);
     e->y.assign(n, 0);
+    e->f.assign(n, 0);
     e->w.resize(n);
     e->uw.resize(n);
     u64 Bw = gl_pow(7, (GL_P - 1) / 192 / n * 5);
@@ -116,57 +117,68 @@ static void run_wg(int threads, uint32_t lds_words, F body) {
     }
 }
 
-template <int B1, int B2, int M>
-static void small_iter(Emu *e, uint32_t k) {
+template <int B1, int B2, int M, int MODE = LNT_SQUARE>
+static void small_iter(Emu *e, uint32_t k, const LntParams &P, bool sub2) {
     uint32_t lds = std::max<uint32_t>(B1 * B2, B1 * (B2 + LNT_PAD));
-    bool sub2 = !e->prp && e->pending;
-    run_wg(B2, lds, [&](int tid, LView L) { lnt_small_body<B1, B2, M>(tid, L, e->P, k, sub2, !e->prp); });
+    run_wg(B2, lds, [&](int tid, LView L) { lnt_small_body<B1, B2, M, MODE>(tid, L, P, k, sub2, !e->prp); });
 }
 
 template <int B1, int B2, int K, int M>
-static void col_fwd(Emu *e, bool sub2) {
+static void col_fwd(Emu *e, bool sub2, const LntParams &P) {
     const Plan &pl = e->pl;
     uint32_t lds = std::max<uint32_t>(pl.R * K, B1 * (B2 + LNT_PAD) * K);
     for (uint32_t b = 0; b < pl.C / K; b++)
-        run_wg(K * B2, lds, [&](int tid, LView L) { lnt_col_fwd_body<B1, B2, K, M>(tid, b, L, e->P, sub2); });
+        run_wg(K * B2, lds, [&](int tid, LView L) { lnt_col_fwd_body<B1, B2, K, M>(tid, b, L, P, sub2); });
 }
 
 template <int B1, int B2, int K>
-static void col_inv(Emu *e) {
+static void col_inv(Emu *e, const LntParams &P) {
     const Plan &pl = e->pl;
     uint32_t lds = B1 * (B2 + LNT_PAD) * K;
     for (uint32_t b = 0; b < pl.C / K; b++)
-        run_wg(K * B2, lds, [&](int tid, LView L) { lnt_col_inv_body<B1, B2, K>(tid, b, L, e->P); });
+        run_wg(K * B2, lds, [&](int tid, LView L) { lnt_col_inv_body<B1, B2, K>(tid, b, L, P); });
 }
 
 template <int B1, int B2, int G>
-static void row_sq(Emu *e) {
+static void row_sq(Emu *e, const LntParams &P, int mode) {
     const Plan &pl = e->pl;
     uint32_t lds = G * B1 * (B2 + LNT_PAD);
     for (uint32_t b = 0; b < pl.R / G; b++)
-        run_wg(G * B2, lds, [&](int tid, LView L) { lnt_row_body<B1, B2, G>(tid, b, L, e->P); });
+        run_wg(G * B2, lds, [&](int tid, LView L) {
+            if (mode == LNT_FORWARD) lnt_row_body<B1, B2, G, LNT_FORWARD>(tid, b, L, P);
+            else if (mode == LNT_MULTIPLY) lnt_row_body<B1, B2, G, LNT_MULTIPLY>(tid, b, L, P);
+            else lnt_row_body<B1, B2, G>(tid, b, L, P);
+        });
 }
 
 // Dispatch tables for the emulated sizes (a subset of the HIP build's).
-static bool run_small(Emu *e, uint32_t k) {
+static bool run_small(Emu *e, uint32_t k, const LntParams &P, bool sub2, int mode = LNT_SQUARE) {
     const Plan &pl = e->pl;
-#define SM(b1, b2)                                                    \
-    if (pl.row.B1 == b1 && pl.row.B2 == b2) {                         \
-        if (pl.M == 4) small_iter<b1, b2, 4>(e, k);                   \
-        else small_iter<b1, b2, 8>(e, k);                             \
-        return true;                                                  \
+#define SM(b1, b2)                                                                          \
+    if (pl.row.B1 == b1 && pl.row.B2 == b2) {                                               \
+        if (mode == LNT_FORWARD) {                                                          \
+            if (pl.M == 4) small_iter<b1, b2, 4, LNT_FORWARD>(e, 1, P, sub2);               \
+            else small_iter<b1, b2, 8, LNT_FORWARD>(e, 1, P, sub2);                         \
+        } else if (mode == LNT_MULTIPLY) {                                                  \
+            if (pl.M == 4) small_iter<b1, b2, 4, LNT_MULTIPLY>(e, 1, P, sub2);              \
+            else small_iter<b1, b2, 8, LNT_MULTIPLY>(e, 1, P, sub2);                        \
+        } else if (pl.M == 4) small_iter<b1, b2, 4>(e, k, P, sub2);                         \
+        else small_iter<b1, b2, 8>(e, k, P, sub2);                                          \
+        return true;                                                                        \
     }
     SM(4, 2) SM(4, 4) SM(8, 4) SM(8, 8) SM(16, 8) SM(16, 16) SM(32, 16) SM(32, 32) SM(64, 32) SM(64, 64)
 #undef SM
     return false;
 }
 
-static bool run_large_once(Emu *e, bool sub2) {
+// one pass: mode LNT_SQUARE = col_fwd, row square, col_inv; LNT_FORWARD = col_fwd, row
+// forward (transform left in P.y); LNT_MULTIPLY = col_fwd, row multiply by P.f, col_inv
+static bool run_large_once(Emu *e, bool sub2, const LntParams &P, int mode = LNT_SQUARE) {
     const Plan &pl = e->pl;
     bool ok = false;
 #define CF(b1, b2, k)                                                            \
     if (!ok && pl.col.B1 == b1 && pl.col.B2 == b2 && pl.K == k) {                \
-        if (pl.M == 4) col_fwd<b1, b2, k, 4>(e, sub2); else col_fwd<b1, b2, k, 8>(e, sub2); \
+        if (pl.M == 4) col_fwd<b1, b2, k, 4>(e, sub2, P); else col_fwd<b1, b2, k, 8>(e, sub2, P); \
         ok = true;                                                               \
     }
     CF(8, 8, 16) CF(16, 8, 16) CF(16, 16, 16) CF(32, 16, 16) CF(32, 32, 16) CF(64, 32, 8) CF(64, 64, 4)
@@ -175,16 +187,17 @@ static bool run_large_once(Emu *e, bool sub2) {
     ok = false;
 #define RW(b1, b2)                                                        \
     if (!ok && pl.row.B1 == b1 && pl.row.B2 == b2) {                      \
-        row_sq<b1, b2, 64 / b2>(e);                                       \
+        row_sq<b1, b2, 64 / b2>(e, P, mode);                              \
         ok = true;                                                        \
     }
     RW(16, 8) RW(16, 16) RW(32, 16) RW(32, 32) RW(64, 32) RW(64, 64)
 #undef RW
     if (!ok) return false;
+    if (mode == LNT_FORWARD) return true;
     ok = false;
 #define CI(b1, b2, k)                                                           \
     if (!ok && pl.col.B1 == b1 && pl.col.B2 == b2 && pl.K == k) {               \
-        col_inv<b1, b2, k>(e);                                                  \
+        col_inv<b1, b2, k>(e, P);                                               \
         ok = true;                                                              \
     }
     CI(8, 8, 16) CI(16, 8, 16) CI(16, 16, 16) CI(32, 16, 16) CI(32, 32, 16) CI(64, 32, 8) CI(64, 64, 4)
@@ -196,11 +209,11 @@ extern "C" int emu_iterate(void *h, uint32_t k) {
     Emu *e = (Emu *)h;
     if (k == 0) return 0;
     if (e->pl.small) {
-        if (!run_small(e, k)) return -1;
+        if (!run_small(e, k, e->P, !e->prp && e->pending)) return -1;
     } else {
         for (uint32_t i = 0; i < k; i++) {
             bool sub2 = !e->prp && e->pending;
-            if (!run_large_once(e, sub2)) return -1;
+            if (!run_large_once(e, sub2, e->P)) return -1;
             e->pending = true;
         }
     }
@@ -228,3 +241,22 @@ extern "C" void emu_set_digits(void *h, const uint64_t *d) {
 
 extern "C" uint32_t emu_levels(void *h) { return ((Emu *)h)->pl.levels; }
 extern "C" uint32_t emu_secured(uint32_t p) { return secured_log2_length(p); }
+
+// dst <- dst * src (same shape), through the forward / multiply kernel modes.
+extern "C" int emu_mul(void *hd, void *hs) {
+    Emu *d = (Emu *)hd, *s = (Emu *)hs;
+    LntParams Ps = s->P;
+    Ps.y = GView<u64>{d->f.data(), (u64)d->f.size()};
+    LntParams Pd = d->P;
+    Pd.f = GView<const u64>{d->f.data(), (u64)d->f.size()};
+    const bool ll = !d->prp;
+    if (d->pl.small) {
+        if (!run_small(s, 1, Ps, ll && s->pending, LNT_FORWARD)) return -1;
+        if (!run_small(d, 1, Pd, ll && d->pending, LNT_MULTIPLY)) return -1;
+    } else {
+        if (!run_large_once(s, ll && s->pending, Ps, LNT_FORWARD)) return -1;
+        if (!run_large_once(d, ll && d->pending, Pd, LNT_MULTIPLY)) return -1;
+    }
+    d->pending = false;
+    return 0;
+}

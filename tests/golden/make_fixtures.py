@@ -61,18 +61,36 @@ def gmp(p: int, mode: str, iters: int, every: int) -> dict:
     return res
 
 
+def gmp_gerbicz(p: int, iters: int, block: int):
+    """res64 of s_k and of the Gerbicz product D = s_0 s_B s_2B ... at every block end."""
+    out = subprocess.run([GMP, str(p), "PRP", str(iters), str(block), "0", str(block)], check=True,
+                         capture_output=True, text=True).stdout
+    res, dres = {}, {}
+    for line in out.strip().splitlines():
+        k, r, d = line.split()
+        res[k], dres[k] = r, d
+    return res, dres
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--long", action="store_true", help="also run the multi-hour p=70M/140M fixtures")
     ap.add_argument("--import-log", nargs=4, metavar=("P", "MODE", "ITERS", "FILE"))
+    ap.add_argument("--block", type=int, default=0, help="with --import-log: the log has Gerbicz columns")
     args = ap.parse_args()
     if args.import_log:
         p, mode, iters, path = args.import_log
         doc = json.load(open(OUT))
-        res = dict(line.split() for line in open(path).read().strip().splitlines())
+        rows = [line.split() for line in open(path).read().strip().splitlines()]
+        res = {r[0]: r[1] for r in rows}
+        case = {"p": int(p), "mode": mode, "iters": int(iters), "res64": res}
+        if args.block:
+            case["block"] = args.block
+            case["dres64"] = {r[0]: r[2] for r in rows}
         doc["cases"] = [c for c in doc["cases"] if not (c["p"] == int(p) and c["mode"] == mode
-                                                        and c["iters"] == int(iters))]
-        doc["cases"].append({"p": int(p), "mode": mode, "iters": int(iters), "res64": res})
+                                                        and c["iters"] == int(iters)
+                                                        and c.get("block", 0) == args.block)]
+        doc["cases"].append(case)
         with open(OUT, "w") as f:
             json.dump(doc, f, indent=1, sort_keys=True)
         print(f"imported {len(res)} checkpoints for p={p} {mode}")
@@ -89,14 +107,24 @@ def main() -> int:
         cases.append({"p": p, "mode": "PRP", "iters": p, "res64": gmp(p, "PRP", p, max(1, p // 4))})
     # the single-test parity workload: LL of M86243, res64 every 10,000 iterations
     cases.append({"p": 86243, "mode": "LL", "iters": 86241, "res64": gmp(86243, "LL", 86241, 1000)})
+    # Gerbicz products (PRP): small path n = 4096 and large path n = 2^17
+    for p, iters, block in ((86243, 86243, 1000), (2000003, 6000, 200)):
+        res, dres = gmp_gerbicz(p, iters, block)
+        cases.append({"p": p, "mode": "PRP", "iters": iters, "block": block, "res64": res, "dres64": dres})
     long_cases = [(70000027, "LL", 20000, 1000), (140000041, "PRP", 10000, 1000)]
     for p, mode, iters, every in long_cases:
         if args.long:
             cases.append({"p": p, "mode": mode, "iters": iters, "res64": gmp(p, mode, iters, every)})
         else:
-            prev = [c for c in old["cases"] if c["p"] == p and c["mode"] == mode]
+            prev = [c for c in old["cases"] if c["p"] == p and c["mode"] == mode and "block" not in c]
             if prev:
                 cases.append(prev[0])
+    prev = [c for c in old["cases"] if c["p"] == 140000041 and "block" in c]
+    if args.long:
+        res, dres = gmp_gerbicz(140000041, 10000, 1000)
+        cases.append({"p": 140000041, "mode": "PRP", "iters": 10000, "block": 1000, "res64": res, "dres64": dres})
+    elif prev:
+        cases.append(prev[0])
     doc = {
         "generator": "tests/golden/make_fixtures.py (oracle/gmp_ll.c, GMP mpz)",
         "definition": "LL: s0=4, s=s^2-2 mod 2^p-1; PRP: s0=3, s=s^2 mod 2^p-1; res64 = low 64 bits "

@@ -138,22 +138,17 @@ def widths(p, n):
 
 
 def digits_of(value, p, n):
-    """Natural digits of an integer value (0 <= value < 2^p)."""
-    w = widths(p, n)
-    out = []
-    for wi in w:
-        out.append(value & ((1 << wi) - 1))
-        value >>= wi
-    return out
+    """Natural digits of an integer value (0 <= value < 2^p), in linear time."""
+    b = bin(value)[2:].zfill(p)[::-1]  # bit i at index i
+    e = [-(-p * i // n) for i in range(n + 1)]
+    return [int(b[e[i]:e[i + 1]][::-1] or "0", 2) for i in range(n)]
 
 
 def value_of(d, p, n):
+    """Integer value of natural digits (each below 2^width), in linear time."""
     w = widths(p, n)
-    v, e = 0, 0
-    for di, wi in zip(d, w):
-        v += di << e
-        e += wi
-    return v
+    parts = [format(di, "b").zfill(wi)[-wi:] if wi else "" for di, wi in zip(d, w)]
+    return int("".join(reversed(parts)) or "0", 2)
 
 
 def ll_python(p, k, s=4, prp=False):

@@ -95,3 +95,29 @@ def test_emulated_large_path_orbit_matches_oracle():
         o.iterate(k)
         e.iterate(k)
         assert e.digits() == o.digits(), k
+
+
+@pytest.mark.parametrize("p,n,prp", [(521, 32, True), (9689, 512, False), (86243, 4096, True),
+                                      (86243, 8192, True), (400009, 32768, False)])
+def test_emulated_product_matches_python(p, n, prp):
+    """The forward / multiply kernel modes (used by the Gerbicz check): x <- x * y mod M_p
+    from pre-carry states with a pending LL -2, then one more squaring."""
+    from helpers import digits_of, value_of
+    L = emu()
+    L.emu_mul.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    rnd = random.Random(n + p)
+    M = (1 << p) - 1
+    a, b = rnd.randrange(M), rnd.randrange(M)
+    ea, eb = Emu(p, n, prp), Emu(p, n, prp)
+    ea.set_digits(digits_of(a, p, n))
+    eb.set_digits(digits_of(b, p, n))
+    ea.iterate(1)
+    eb.iterate(2)
+    c = 0 if prp else 2
+    a = (a * a - c) % M
+    for _ in range(2):
+        b = (b * b - c) % M
+    assert L.emu_mul(ea.h, eb.h) == 0
+    assert value_of(ea.digits(), p, n) == a * b % M
+    ea.iterate(1)
+    assert value_of(ea.digits(), p, n) == ((a * b % M) ** 2 - c) % M
