@@ -5,7 +5,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -58,6 +60,7 @@ struct lnt_ctx {
     std::vector<uint8_t> wid;
     u64 *x = nullptr, *y = nullptr, *w = nullptr, *uw = nullptr;
     u64 *tw4 = nullptr, *itw4 = nullptr, *twr = nullptr, *itwr = nullptr, *twc = nullptr, *itwc = nullptr;
+    u64 *twr2 = nullptr, *itwr2 = nullptr;
     u64 *f = nullptr;  // transform of the second operand of a product (allocated on first use)
     LntParams P{};
     std::string err;
@@ -99,7 +102,7 @@ static int fail(lnt_ctx *c, int code, const std::string &msg) {
 static void free_buffers(lnt_ctx *c) {
     u64 **bufs[] = {&c->x,      &c->y,      &c->w,      &c->uw,      &c->tw4,     &c->itw4,   &c->twr,
                     &c->itwr,   &c->twc,    &c->itwc,   &c->f,       &c->g.U0.x,  &c->g.D.x,  &c->g.Dprev.x,
-                    &c->g.E.x,  &c->g.Uv.x, &c->g.Dv.x};
+                    &c->g.E.x,  &c->g.Uv.x, &c->g.Dv.x, &c->twr2, &c->itwr2};
     for (u64 **b : bufs)
         if (*b) {
             (void)hipFree(*b);
@@ -135,10 +138,10 @@ static int build_tables(lnt_ctx *c) {
     hipLaunchKernelGGL(lnt_k_weights, dim3((n + 255) / 256), dim3(256), 0, c->stream, Bw, gl_inv(Bw), ninv, c->p,
                        n, c->w, c->uw);
     u64 omr = gl_pow(554, (GL_P - 1) / Lr);
-    hipLaunchKernelGGL(lnt_k_line_tw, dim3((Lr + 255) / 256), dim3(256), 0, c->stream, omr, pl.row.B1, pl.row.B2, Lr,
-                       c->twr);
-    hipLaunchKernelGGL(lnt_k_line_tw, dim3((Lr + 255) / 256), dim3(256), 0, c->stream, gl_inv(omr), pl.row.B1,
-                       pl.row.B2, Lr, c->itwr);
+    const u32 rB1 = pl.row3 ? pl.row3 : pl.row.B1, rB2 = pl.row3 ? pl.row3 * pl.row3 : pl.row.B2;
+    hipLaunchKernelGGL(lnt_k_line_tw, dim3((Lr + 255) / 256), dim3(256), 0, c->stream, omr, rB1, rB2, Lr, c->twr);
+    hipLaunchKernelGGL(lnt_k_line_tw, dim3((Lr + 255) / 256), dim3(256), 0, c->stream, gl_inv(omr), rB1, rB2, Lr,
+                       c->itwr);
     if (!pl.small) {
         u64 omc = gl_pow(554, (GL_P - 1) / pl.R);
         hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, c->stream, omc, pl.col.B1,
@@ -151,8 +154,20 @@ static int build_tables(lnt_ctx *c) {
         hipLaunchKernelGGL(lnt_k_tw4, dim3((n + 255) / 256), dim3(256), 0, c->stream, gl_inv(omn), pl.C, pl.col.B1,
                            pl.col.B2, n, c->itw4);
     }
+    if (pl.row3) {
+        const u32 L2 = pl.row3 * pl.row3;
+        HIPCHK(c, hipMalloc(&c->twr2, (size_t)L2 * 8), "hipMalloc (row twiddles)");
+        HIPCHK(c, hipMalloc(&c->itwr2, (size_t)L2 * 8), "hipMalloc (row twiddles)");
+        u64 om2 = gl_pow(554, (GL_P - 1) / L2);
+        hipLaunchKernelGGL(lnt_k_line_tw, dim3((L2 + 255) / 256), dim3(256), 0, c->stream, om2, pl.row3, pl.row3, L2,
+                           c->twr2);
+        hipLaunchKernelGGL(lnt_k_line_tw, dim3((L2 + 255) / 256), dim3(256), 0, c->stream, gl_inv(om2), pl.row3,
+                           pl.row3, L2, c->itwr2);
+    }
     HIPCHK(c, hipGetLastError(), "table kernels launch");
     LntParams &P = c->P;
+    P.twr2 = view<const u64>(c->twr2, pl.row3 ? pl.row3 * pl.row3 : 0);
+    P.itwr2 = view<const u64>(c->itwr2, pl.row3 ? pl.row3 * pl.row3 : 0);
     P.x = view<u64>(c->x, n);
     P.y = view<u64>(c->y, n);
     P.w = view<const u64>(c->w, n);
@@ -273,12 +288,47 @@ static int canonical_of(lnt_ctx *c, const u64 *buf, bool pending, std::vector<ui
     return LNT_OK;
 }
 
-static int make_context(int device, uint32_t p, uint64_t n, bool prp, uint32_t force_lc, lnt_ctx **out) {
+// Plans the tuner picked, per (device, n); new contexts of that length start from them.
+static std::mutex g_tuned_mu;
+static std::map<std::pair<int, uint32_t>, PlanOpts> g_tuned;
+
+// LNT_PLAN="lc=12,K=8,row3=1" overrides the plan of new contexts (experiments, tests).
+static bool env_plan(PlanOpts &o) {
+    const char *e = std::getenv("LNT_PLAN");
+    if (!e || !*e) return false;
+    std::string s = e;
+    size_t pos = 0;
+    while (pos < s.size()) {
+        size_t q = s.find(',', pos);
+        std::string kv = s.substr(pos, q == std::string::npos ? std::string::npos : q - pos);
+        size_t eq = kv.find('=');
+        if (eq != std::string::npos) {
+            std::string k = kv.substr(0, eq);
+            int v = std::atoi(kv.c_str() + eq + 1);
+            if (k == "lc") o.lc = (uint32_t)v;
+            if (k == "K") o.K = (uint32_t)v;
+            if (k == "row3") o.row3 = v;
+        }
+        if (q == std::string::npos) break;
+        pos = q + 1;
+    }
+    return true;
+}
+
+static int make_context(int device, uint32_t p, uint64_t n, bool prp, const PlanOpts *opts, lnt_ctx **out) {
     lnt_ctx *c = new lnt_ctx;
     c->device = device;
     c->p = p;
     c->prp = prp;
-    std::string why = make_plan(p, n, c->pl, force_lc);
+    PlanOpts o;
+    if (opts) o = *opts;
+    else if (!env_plan(o)) {
+        std::lock_guard<std::mutex> lk(g_tuned_mu);
+        auto it = g_tuned.find({device, (uint32_t)n});
+        if (it != g_tuned.end()) o = it->second;
+    }
+    std::string why = make_plan(p, n, c->pl, o);
+    if (!why.empty() && !opts) why = make_plan(p, n, c->pl, PlanOpts());  // stale tuned entry
     if (!why.empty()) {
         delete c;
         return fail(nullptr, LNT_UNSUPPORTED, why);
@@ -347,7 +397,7 @@ int lnt_create(int device, uint32_t p, const uint32_t *factors, int nfactors, in
             n *= factors[i];
             if (n > (1ull << 32)) return fail(nullptr, LNT_UNSUPPORTED, "NTT length too large");
         }
-    return make_context(device, p, n, mode == LNT_MODE_PRP, 0, out);
+    return make_context(device, p, n, mode == LNT_MODE_PRP, nullptr, out);
 }
 
 int lnt_set_digits(lnt_ctx *c, const uint64_t *d, size_t n) {
@@ -419,12 +469,15 @@ uint32_t lnt_secured_log2_length(uint32_t p) { return secured_log2_length(p); }
 uint32_t lnt_length(const lnt_ctx *c) { return c ? c->pl.n : 0; }
 
 static std::string describe(const Plan &pl) {
-    char b[256];
+    char b[320];
     if (pl.small) {
         std::snprintf(b, sizeof b, "LNT_1WG;lnt_k_small<%u,%u>", pl.row.B1, pl.row.B2);
     } else {
-        std::snprintf(b, sizeof b, "LNT_2PASS_%ux%u;lnt_k_col_fwd<%u,%u,%u>;lnt_k_row<%u,%u>;lnt_k_col_inv<%u,%u,%u>",
-                      pl.R, pl.C, pl.col.B1, pl.col.B2, pl.K, pl.row.B1, pl.row.B2, pl.col.B1, pl.col.B2, pl.K);
+        char row[64];
+        if (pl.row3) std::snprintf(row, sizeof row, "lnt_k_row3<%u,%u>", pl.row3, pl.G);
+        else std::snprintf(row, sizeof row, "lnt_k_row<%u,%u>", pl.row.B1, pl.row.B2);
+        std::snprintf(b, sizeof b, "LNT_2PASS_%ux%u;lnt_k_col_fwd<%u,%u,%u>;%s;lnt_k_col_inv<%u,%u,%u>", pl.R, pl.C,
+                      pl.col.B1, pl.col.B2, pl.K, row, pl.col.B1, pl.col.B2, pl.K);
     }
     return b;
 }
@@ -503,42 +556,65 @@ int lnt_tune(lnt_ctx *c, int loops) {
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     const bool start_pending = c->pending;
     struct Cand {
-        uint32_t lc;
+        PlanOpts o;
         float ms;
         std::vector<uint64_t> out;
     };
+    // candidates: every row/column split, column tile and row variant this build has
+    std::vector<PlanOpts> opts;
+    for (uint32_t lc = 7; lc <= 12; lc++)
+        for (uint32_t K : {16u, 8u, 4u})
+            for (int r3 : {0, 1}) {
+                PlanOpts o;
+                o.lc = lc;
+                o.K = K;
+                o.row3 = r3;
+                Plan tmp;
+                if (make_plan(c->p, n, tmp, o).empty()) opts.push_back(o);
+            }
     std::vector<Cand> cands;
-    for (uint32_t lc = 7; lc <= 12; lc++) {
-        uint32_t lr = c->pl.log_n - lc;
-        if (lr < 6 || lr > 12) continue;
+    const uint32_t warm = std::max(2, loops / 10);
+    for (const PlanOpts &o : opts) {
         lnt_ctx *t = nullptr;
-        if (make_context(c->device, c->p, n, c->prp, lc, &t) != LNT_OK) continue;
+        if (make_context(c->device, c->p, n, c->prp, &o, &t) != LNT_OK) continue;
         HIPCHK(c, hipMemcpyAsync(t->x, start.data(), (size_t)n * 8, hipMemcpyHostToDevice, t->stream), "hipMemcpy");
         t->pending = start_pending;
         t->pristine = false;
         float ms = 0;
+        // warm-up on a copy of the state, then time `loops` iterations from the start state
+        if (run_iterations(t, warm) != LNT_OK || lnt_sync(t, nullptr) != LNT_OK) {
+            lnt_destroy(t);
+            continue;
+        }
+        HIPCHK(c, hipMemcpyAsync(t->x, start.data(), (size_t)n * 8, hipMemcpyHostToDevice, t->stream), "hipMemcpy");
+        t->pending = start_pending;
         lnt_sync(t, nullptr);
         if (run_iterations(t, (uint32_t)loops) != LNT_OK || lnt_sync(t, &ms) != LNT_OK) {
             lnt_destroy(t);
             continue;
         }
-        Cand cd{lc, ms, std::vector<uint64_t>(n)};
+        Cand cd{o, ms, std::vector<uint64_t>(n)};
         read_digits(t, cd.out.data());
         lnt_destroy(t);
         cands.push_back(std::move(cd));
     }
     if (cands.empty()) return fail(c, LNT_DEVICE_ERROR, "tuner: no candidate ran");
-    // every candidate must agree with the others (the current split is among them)
+    // every candidate must agree with the others before its time counts
     for (auto &cd : cands)
         if (cd.out != cands[0].out) return fail(c, LNT_DEVICE_ERROR, "tuner: candidates disagree on the residue");
     const Cand *best = &cands[0];
     for (auto &cd : cands)
         if (cd.ms < best->ms) best = &cd;
-    uint32_t cur_lc = ilog2(c->pl.C);
-    if (best->lc == cur_lc) return LNT_OK;
-    // rebuild this context on the winning split, keeping the residue
+    {
+        std::lock_guard<std::mutex> lk(g_tuned_mu);
+        g_tuned[{c->device, n}] = best->o;
+    }
+    Plan bp;
+    make_plan(c->p, n, bp, best->o);
+    if (bp.C == c->pl.C && bp.K == c->pl.K && bp.row3 == c->pl.row3) return LNT_OK;
+    // rebuild this context on the winning plan, keeping the residue
     lnt_ctx *nc = nullptr;
-    int rc = make_context(c->device, c->p, n, c->prp, best->lc, &nc);
+    int rc = make_context(c->device, c->p, n, c->prp, &best->o, &nc);
     if (rc != LNT_OK) return fail(c, rc, lnt_last_error(nullptr));
     HIPCHK(c, hipMemcpyAsync(nc->x, start.data(), (size_t)n * 8, hipMemcpyHostToDevice, nc->stream), "hipMemcpy");
     HIPCHK(c, hipStreamSynchronize(nc->stream), "hipStreamSynchronize");
@@ -551,6 +627,8 @@ int lnt_tune(lnt_ctx *c, int loops) {
     std::swap(c->itw4, nc->itw4);
     std::swap(c->twr, nc->twr);
     std::swap(c->itwr, nc->itwr);
+    std::swap(c->twr2, nc->twr2);
+    std::swap(c->itwr2, nc->itwr2);
     std::swap(c->twc, nc->twc);
     std::swap(c->itwc, nc->itwc);
     std::swap(c->P, nc->P);
@@ -708,5 +786,226 @@ int lnt_gerbicz_stats(lnt_ctx *c, uint64_t *out, int cap) {
     out[4] = g.ck_res64;
     return LNT_OK;
 }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+// Many independent tests advanced together: members with the same plan shape form a
+// group that runs as one batched launch per kernel (per-test parameters in a device
+// array); one iteration of every group is captured as a hipGraph whose groups are
+// parallel branches, and each lnt_batch_iterate step replays it.
+
+struct lnt_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_mark = nullptr;
+    std::vector<lnt_ctx *> members;
+    struct Group {
+        Plan pl;
+        bool ll = false;
+        std::vector<lnt_ctx *> ctxs;
+        std::vector<LntParams> h;
+        LntParams *d = nullptr;
+        hipStream_t side = nullptr;
+        hipEvent_t join = nullptr;
+    };
+    std::vector<Group> groups;
+    hipEvent_t fork = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool dirty = true;
+    std::string err;
+};
+
+static int bfail(lnt_batch *b, int code, const std::string &m) {
+    if (b) b->err = m;
+    return fail(nullptr, code, m);
+}
+
+#define BHIPCHK(b, call, what)                                                                           \
+    do {                                                                                                  \
+        hipError_t e_ = (call);                                                                           \
+        if (e_ != hipSuccess)                                                                             \
+            return bfail((b), e_ == hipErrorOutOfMemory ? LNT_OUT_OF_MEMORY : LNT_DEVICE_ERROR,           \
+                         std::string(what) + " (" + hipGetErrorString(e_) + ")");                         \
+    } while (0)
+
+static bool same_group(const Plan &a, const Plan &b) {
+    return a.small == b.small && a.n == b.n && a.C == b.C && a.K == b.K && a.row3 == b.row3 && a.M == b.M;
+}
+
+static void batch_release(lnt_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    for (auto &g : b->groups) {
+        if (g.d) (void)hipFree(g.d);
+        if (g.side) (void)hipStreamDestroy(g.side);
+        if (g.join) (void)hipEventDestroy(g.join);
+    }
+    b->groups.clear();
+}
+
+// Launch one iteration of group g on stream s.
+static bool batch_group_step(lnt_batch::Group &g, hipStream_t s) {
+    const u32 cnt = (u32)g.ctxs.size();
+    if (g.pl.small) return lnt_launch_batch_small(g.pl, g.d, cnt, 1, g.ll, s);
+    return lnt_launch_batch_col_fwd(g.pl, g.d, cnt, s) && lnt_launch_batch_row(g.pl, g.d, cnt, s) &&
+           lnt_launch_batch_col_inv(g.pl, g.d, cnt, s);
+}
+
+static int batch_build(lnt_batch *b) {
+    batch_release(b);
+    for (lnt_ctx *c : b->members) {
+        lnt_batch::Group *grp = nullptr;
+        for (auto &g : b->groups)
+            if (same_group(g.pl, c->pl) && g.ll == !c->prp) grp = &g;
+        if (!grp) {
+            b->groups.emplace_back();
+            grp = &b->groups.back();
+            grp->pl = c->pl;
+            grp->ll = !c->prp;
+        }
+        grp->ctxs.push_back(c);
+    }
+    for (auto &g : b->groups) {
+        g.h.clear();
+        for (lnt_ctx *c : g.ctxs) {
+            LntParams P = c->P;
+            P.sub2 = (!c->prp && c->pending) ? 1u : 0u;
+            g.h.push_back(P);
+        }
+        BHIPCHK(b, hipMalloc(&g.d, g.h.size() * sizeof(LntParams)), "hipMalloc (batch parameters)");
+        BHIPCHK(b, hipMemcpyAsync(g.d, g.h.data(), g.h.size() * sizeof(LntParams), hipMemcpyHostToDevice, b->stream),
+                "hipMemcpy (batch parameters)");
+        BHIPCHK(b, hipStreamCreateWithFlags(&g.side, hipStreamNonBlocking), "hipStreamCreate");
+        BHIPCHK(b, hipEventCreateWithFlags(&g.join, hipEventDisableTiming), "hipEventCreate");
+    }
+    BHIPCHK(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    // capture: fork every group onto its own stream, join back
+    BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    bool ok = true;
+    if (hipEventRecord(b->fork, b->stream) != hipSuccess) ok = false;
+    for (auto &g : b->groups) {
+        if (!ok) break;
+        ok = hipStreamWaitEvent(g.side, b->fork, 0) == hipSuccess && batch_group_step(g, g.side) &&
+             hipEventRecord(g.join, g.side) == hipSuccess && hipStreamWaitEvent(b->stream, g.join, 0) == hipSuccess;
+    }
+    hipError_t ec = hipStreamEndCapture(b->stream, &b->graph);
+    if (!ok || ec != hipSuccess) return bfail(b, LNT_DEVICE_ERROR, "batch graph capture failed");
+    BHIPCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    b->dirty = false;
+    return LNT_OK;
+}
+
+// After an iteration every LL member owes its "-2": refresh the flags where they changed.
+static int batch_mark_pending(lnt_batch *b) {
+    for (auto &g : b->groups) {
+        bool changed = false;
+        for (size_t i = 0; i < g.ctxs.size(); i++) {
+            lnt_ctx *c = g.ctxs[i];
+            c->pending = true;
+            c->pristine = false;
+            u32 want = c->prp ? 0u : 1u;
+            if (g.h[i].sub2 != want) {
+                g.h[i].sub2 = want;
+                changed = true;
+            }
+        }
+        if (changed)
+            BHIPCHK(b, hipMemcpyAsync(g.d, g.h.data(), g.h.size() * sizeof(LntParams), hipMemcpyHostToDevice, b->stream),
+                    "hipMemcpy (batch parameters)");
+    }
+    return LNT_OK;
+}
+
+extern "C" {
+
+int lnt_batch_create(int device, lnt_batch **out) {
+    if (!out) return fail(nullptr, LNT_INVALID_ARGUMENT, "out is NULL");
+    lnt_batch *b = new lnt_batch;
+    b->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev_mark) != hipSuccess || hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventRecord(b->ev_mark, b->stream) != hipSuccess) {
+        delete b;
+        return fail(nullptr, LNT_DEVICE_ERROR, "lnt_batch_create: HIP stream/event set-up failed");
+    }
+    *out = b;
+    return LNT_OK;
+}
+
+int lnt_batch_add(lnt_batch *b, lnt_ctx *c) {
+    if (!b || !c) return bfail(b, LNT_INVALID_ARGUMENT, "NULL argument");
+    if (c->device != b->device) return bfail(b, LNT_INVALID_ARGUMENT, "context is on another device");
+    for (lnt_ctx *m : b->members)
+        if (m == c) return bfail(b, LNT_INVALID_ARGUMENT, "context is already in the batch");
+    BHIPCHK(b, hipSetDevice(b->device), "hipSetDevice");
+    BHIPCHK(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    b->members.push_back(c);
+    b->dirty = true;
+    return LNT_OK;
+}
+
+int lnt_batch_remove(lnt_batch *b, lnt_ctx *c) {
+    if (!b || !c) return bfail(b, LNT_INVALID_ARGUMENT, "NULL argument");
+    BHIPCHK(b, hipSetDevice(b->device), "hipSetDevice");
+    BHIPCHK(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    for (size_t i = 0; i < b->members.size(); i++)
+        if (b->members[i] == c) {
+            b->members.erase(b->members.begin() + i);
+            b->dirty = true;
+            return LNT_OK;
+        }
+    return bfail(b, LNT_INVALID_ARGUMENT, "context is not in the batch");
+}
+
+int lnt_batch_size(const lnt_batch *b) { return b ? (int)b->members.size() : 0; }
+
+int lnt_batch_iterate(lnt_batch *b, uint32_t k) {
+    if (!b) return fail(nullptr, LNT_INVALID_ARGUMENT, "NULL batch");
+    if (k == 0 || b->members.empty()) return LNT_OK;
+    BHIPCHK(b, hipSetDevice(b->device), "hipSetDevice");
+    for (lnt_ctx *c : b->members) BHIPCHK(b, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    int rc;
+    for (auto &g : b->groups)  // a member re-planned (lnt_tune) since the graph was built
+        for (size_t i = 0; i < g.ctxs.size() && !b->dirty; i++)
+            if (g.h[i].x.p != g.ctxs[i]->P.x.p || g.h[i].twr.p != g.ctxs[i]->P.twr.p) b->dirty = true;
+    if (b->dirty && (rc = batch_build(b))) return rc;
+    for (uint32_t i = 0; i < k; i++) {
+        BHIPCHK(b, hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch");
+        if (i == 0 && (rc = batch_mark_pending(b))) return rc;
+    }
+    return LNT_OK;
+}
+
+int lnt_batch_sync(lnt_batch *b, float *elapsed_ms) {
+    if (!b) return fail(nullptr, LNT_INVALID_ARGUMENT, "NULL batch");
+    BHIPCHK(b, hipSetDevice(b->device), "hipSetDevice");
+    hipEvent_t now;
+    BHIPCHK(b, hipEventCreate(&now), "hipEventCreate");
+    BHIPCHK(b, hipEventRecord(now, b->stream), "hipEventRecord");
+    BHIPCHK(b, hipEventSynchronize(now), "hipEventSynchronize");
+    float ms = 0;
+    BHIPCHK(b, hipEventElapsedTime(&ms, b->ev_mark, now), "hipEventElapsedTime");
+    (void)hipEventDestroy(b->ev_mark);
+    b->ev_mark = now;
+    if (elapsed_ms) *elapsed_ms = ms;
+    return LNT_OK;
+}
+
+void lnt_batch_destroy(lnt_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_release(b);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->ev_mark) (void)hipEventDestroy(b->ev_mark);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char *lnt_batch_last_error(const lnt_batch *b) { return b ? b->err.c_str() : lnt_last_error(nullptr); }
 
 }  // extern "C"

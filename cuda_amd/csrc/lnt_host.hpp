@@ -56,6 +56,7 @@ struct Plan {
     Line row, col;          // row line (also the small path's line), column line
     uint32_t K = 0;         // columns per column tile
     uint32_t G = 0;         // rows per row workgroup
+    uint32_t row3 = 0;      // 3-phase rows: C = row3^3 (0: 2-phase rows)
     uint32_t M = 0;         // carry halo (compile-time), >= levels
     uint32_t levels = 0;    // carry levels
 };
@@ -96,8 +97,25 @@ inline bool split_for(uint32_t log_n, uint32_t &lc, uint32_t &lr) {
     return lr <= 12 && lr >= 6;
 }
 
+// Choices the tuner may vary (0 = default).
+struct PlanOpts {
+    uint32_t lc = 0;    // log2 of the row length
+    uint32_t K = 0;     // column tile width
+    int row3 = -1;      // 1: 3-phase rows when C is a cube, 0: 2-phase, -1: default
+};
+
+// Column tile widths that fit the column kernels' LDS (R * K words of 8 bytes <= 128 KiB).
+inline bool col_tile_ok(uint32_t R, uint32_t K) {
+    static const uint32_t shapes[][2] = {{64, 16}, {128, 16}, {256, 16}, {512, 16}, {1024, 16}, {1024, 8},
+                                         {2048, 8}, {2048, 4}, {4096, 4}};
+    for (auto &s : shapes)
+        if (s[0] == R && s[1] == K) return true;
+    return false;
+}
+
 // Build the plan; returns an empty string on success, else the reason.
-inline std::string make_plan(uint32_t p, uint64_t n, Plan &pl, uint32_t force_lc = 0) {
+inline std::string make_plan(uint32_t p, uint64_t n, Plan &pl, PlanOpts opts = PlanOpts()) {
+    const uint32_t force_lc = opts.lc;
     if (n < 8 || (n & (n - 1))) return "NTT length must be a power of two >= 8";
     pl.n = (uint32_t)n;
     pl.log_n = ilog2(n);
@@ -126,7 +144,16 @@ inline std::string make_plan(uint32_t p, uint64_t n, Plan &pl, uint32_t force_lc
     pl.row = line_for(lc);
     pl.col = line_for(lr);
     pl.K = pl.R >= 4096 ? 4 : pl.R >= 2048 ? 8 : 16;
+    if (opts.K) {
+        if (!col_tile_ok(pl.R, opts.K)) return "unsupported column tile";
+        pl.K = opts.K;
+    }
     pl.G = 64 / pl.row.B2;
+    if (opts.row3 == 1) {
+        if (pl.C != 4096 && pl.C != 512) return "3-phase rows need C = 512 or 4096";
+        pl.row3 = pl.C == 4096 ? 16 : 8;
+        pl.G = pl.C == 4096 ? 1 : 4;
+    }
     return "";
 }
 

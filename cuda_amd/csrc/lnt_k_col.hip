@@ -18,7 +18,8 @@ static bool go_col_inv(const lnt::Plan &pl, const LntParams &P, hipStream_t s) {
     return true;
 }
 
-#define COL_SHAPES(X) X(8, 8, 16) X(16, 8, 16) X(16, 16, 16) X(32, 16, 16) X(32, 32, 16) X(64, 32, 8) X(64, 64, 4)
+#define COL_SHAPES(X) \
+    X(8, 8, 16) X(16, 8, 16) X(16, 16, 16) X(32, 16, 16) X(32, 32, 16) X(32, 32, 8) X(64, 32, 8) X(64, 32, 4) X(64, 64, 4)
 
 bool lnt_launch_col_fwd(const lnt::Plan &pl, const LntParams &P, bool sub2, hipStream_t s) {
 #define CF(b1, b2, k) if (pl.col.B1 == b1 && pl.col.B2 == b2 && pl.K == k) return go_col_fwd<b1, b2, k>(pl, P, sub2, s);
@@ -29,6 +30,33 @@ bool lnt_launch_col_fwd(const lnt::Plan &pl, const LntParams &P, bool sub2, hipS
 
 bool lnt_launch_col_inv(const lnt::Plan &pl, const LntParams &P, hipStream_t s) {
 #define CI(b1, b2, k) if (pl.col.B1 == b1 && pl.col.B2 == b2 && pl.K == k) return go_col_inv<b1, b2, k>(pl, P, s);
+    COL_SHAPES(CI)
+#undef CI
+    return false;
+}
+
+bool lnt_launch_batch_col_fwd(const lnt::Plan &pl, const LntParams *Ps, u32 count, hipStream_t s) {
+#define CF(b1, b2, k)                                                                                     \
+    if (pl.col.B1 == b1 && pl.col.B2 == b2 && pl.K == k) {                                                \
+        const u32 wgs = pl.C / k;                                                                         \
+        if (pl.M == 4)                                                                                    \
+            hipLaunchKernelGGL((lnt_kb_col_fwd<b1, b2, k, 4>), dim3(wgs * count), dim3(k * b2), 0, s, Ps, wgs); \
+        else                                                                                              \
+            hipLaunchKernelGGL((lnt_kb_col_fwd<b1, b2, k, 8>), dim3(wgs * count), dim3(k * b2), 0, s, Ps, wgs); \
+        return true;                                                                                      \
+    }
+    COL_SHAPES(CF)
+#undef CF
+    return false;
+}
+
+bool lnt_launch_batch_col_inv(const lnt::Plan &pl, const LntParams *Ps, u32 count, hipStream_t s) {
+#define CI(b1, b2, k)                                                                                   \
+    if (pl.col.B1 == b1 && pl.col.B2 == b2 && pl.K == k) {                                              \
+        const u32 wgs = pl.C / k;                                                                       \
+        hipLaunchKernelGGL((lnt_kb_col_inv<b1, b2, k>), dim3(wgs * count), dim3(k * b2), 0, s, Ps, wgs); \
+        return true;                                                                                    \
+    }
     COL_SHAPES(CI)
 #undef CI
     return false;

@@ -14,8 +14,45 @@ static bool go_row(const lnt::Plan &pl, const LntParams &P, hipStream_t s, int m
     return true;
 }
 
+template <int B, int G>
+static bool go_row3(const lnt::Plan &pl, const LntParams &P, hipStream_t s, int mode) {
+    dim3 grid(pl.R / G), block(G * B * B);
+    if (mode == LNT_FORWARD)
+        hipLaunchKernelGGL((lnt_k_row3<B, G, LNT_FORWARD>), grid, block, 0, s, P);
+    else if (mode == LNT_MULTIPLY)
+        hipLaunchKernelGGL((lnt_k_row3<B, G, LNT_MULTIPLY>), grid, block, 0, s, P);
+    else
+        hipLaunchKernelGGL((lnt_k_row3<B, G, LNT_SQUARE>), grid, block, 0, s, P);
+    return true;
+}
+
 bool lnt_launch_row(const lnt::Plan &pl, const LntParams &P, hipStream_t s, int mode) {
+    if (pl.row3 == 16) return go_row3<16, 1>(pl, P, s, mode);
+    if (pl.row3 == 8) return go_row3<8, 4>(pl, P, s, mode);
 #define RW(b1, b2) if (pl.row.B1 == b1 && pl.row.B2 == b2) return go_row<b1, b2>(pl, P, s, mode);
+    RW(16, 8) RW(16, 16) RW(32, 16) RW(32, 32) RW(64, 32) RW(64, 64)
+#undef RW
+    return false;
+}
+
+bool lnt_launch_batch_row(const lnt::Plan &pl, const LntParams *Ps, u32 count, hipStream_t s) {
+    if (pl.row3 == 16) {
+        const u32 wgs = pl.R;
+        hipLaunchKernelGGL((lnt_kb_row3<16, 1>), dim3(wgs * count), dim3(256), 0, s, Ps, wgs);
+        return true;
+    }
+    if (pl.row3 == 8) {
+        const u32 wgs = pl.R / 4;
+        hipLaunchKernelGGL((lnt_kb_row3<8, 4>), dim3(wgs * count), dim3(256), 0, s, Ps, wgs);
+        return true;
+    }
+#define RW(b1, b2)                                                                                   \
+    if (pl.row.B1 == b1 && pl.row.B2 == b2) {                                                        \
+        constexpr int G = 64 / b2;                                                                   \
+        const u32 wgs = pl.R / G;                                                                    \
+        hipLaunchKernelGGL((lnt_kb_row<b1, b2, G>), dim3(wgs * count), dim3(G * b2), 0, s, Ps, wgs); \
+        return true;                                                                                 \
+    }
     RW(16, 8) RW(16, 16) RW(32, 16) RW(32, 32) RW(64, 32) RW(64, 64)
 #undef RW
     return false;

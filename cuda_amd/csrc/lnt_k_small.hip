@@ -27,3 +27,20 @@ bool lnt_launch_small(const lnt::Plan &pl, const LntParams &P, u32 iters, bool s
 #undef SM
     return false;
 }
+
+template <int B1, int B2>
+static bool go_batch_small(const lnt::Plan &pl, const LntParams *Ps, u32 count, u32 iters, bool ll, hipStream_t s) {
+    dim3 grid(count), block(B2);
+    if (pl.M == 4)
+        hipLaunchKernelGGL((lnt_kb_small<B1, B2, 4>), grid, block, 0, s, Ps, iters, (int)ll);
+    else
+        hipLaunchKernelGGL((lnt_kb_small<B1, B2, 8>), grid, block, 0, s, Ps, iters, (int)ll);
+    return true;
+}
+
+bool lnt_launch_batch_small(const lnt::Plan &pl, const LntParams *Ps, u32 count, u32 iters, bool ll, hipStream_t s) {
+#define SM(b1, b2) if (pl.row.B1 == b1 && pl.row.B2 == b2) return go_batch_small<b1, b2>(pl, Ps, count, iters, ll, s);
+    SM(4, 2) SM(4, 4) SM(8, 4) SM(8, 8) SM(16, 8) SM(16, 16) SM(32, 16) SM(32, 32) SM(64, 32) SM(64, 64)
+#undef SM
+    return false;
+}

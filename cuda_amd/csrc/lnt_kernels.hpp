@@ -45,10 +45,12 @@ struct LntParams {
     GView<const u64> w, uw;  // IBDWT weights, unweights * n^-1 (natural digit order)
     GView<const u64> tw4, itw4;  // 4-step twiddles (large path), natural slots
     GView<const u64> twr, itwr;  // row-line twiddles (C words)
+    GView<const u64> twr2, itwr2;  // second-level row twiddles (3-phase rows: B^2 words)
     GView<const u64> twc, itwc;  // column-line twiddles (R words)
     GView<const u64> f;      // n words: the other operand of a product, transform domain
     u32 p, n, log_n, C, R;
     u32 levels;              // carry levels m
+    u32 sub2;                // batched launches: this test's LL "-2" is due at the next carry
 };
 
 // e_g = ceil(p g / n) for 0 <= g <= n (bit offset of digit g).
@@ -256,6 +258,87 @@ LNT_KFN void lnt_row_body(int tid, int bid, LView lds, const LntParams &P) {
     dft_inv<B1>(a);
 #pragma unroll
     for (int j = 0; j < B1; j++) P.y[base + t + (u64)B2 * j] = a[j];
+}
+
+// ------------------------------------------------------------------ 3-phase row pass
+// Rows of C = B^3 words, B^2 threads per row each holding B words (more waves per SIMD
+// than the 2-phase row: B registers instead of B1 = 64).  With i = i1 + B^2 i2
+// (i1 < B^2) and i1 = a + B b:
+//   phase 1: thread i1: B-point DFT over i2 -> k2, twiddle omega_C^(i1 k2)      (twr)
+//   phase 2: thread (k2, a): B-point DFT over b -> kb, twiddle omega_B^2^(a kb) (twr2)
+//   phase 3: thread (k2, kb): B-point DFT over a -> ka; X[B (B ka + kb) + k2]
+// then the pointwise square (or the product modes of lnt_row_body) and the inverse in
+// reverse.  Within the LDS row of k2 a word (r, a) sits at r * (B + 1) + a, so every
+// phase reads and writes conflict-free strides.  Transform-domain slot for MODE 1/2:
+// base + k2 * B^2 + kb * B + i (thread-contiguous).
+template <int B, int G, int MODE = LNT_SQUARE>
+LNT_KFN void lnt_row3_body(int tid, int bid, LView lds, const LntParams &P) {
+    constexpr int T = B * B;        // threads per row
+    constexpr int RS = B * (B + 1); // LDS words per k2 row
+    constexpr int LINE = B * RS;    // LDS words per row
+    const int t = tid % T, g = tid / T;
+    const u64 base = (u64)P.C * ((u64)bid * G + g);
+    const u32 lb = (u32)(g * LINE);
+    const int hi = t / B, lo = t % B;  // phases 2/3: (k2, a) or (k2, kb)
+    u64 a[B];
+    // phase 1
+#pragma unroll
+    for (int j = 0; j < B; j++) a[j] = P.y[base + t + (u64)T * j];
+    dft_fwd<B>(a);
+#pragma unroll
+    for (int rho = 1; rho < B; rho++) a[rho] = gl_mul(a[rho], P.twr[(u64)rho * T + t]);
+#pragma unroll
+    for (int rho = 0; rho < B; rho++) lds[lb + (u32)(brev_c<B>(rho) * RS + (t / B) * (B + 1) + t % B)] = a[rho];
+    LNT_SYNC();
+    // phase 2: thread (k2 = hi, a = lo) over b
+#pragma unroll
+    for (int b = 0; b < B; b++) a[b] = lds[lb + (u32)(hi * RS + b * (B + 1) + lo)];
+    // (this thread rewrites exactly the slots it just read: no barrier needed)
+    dft_fwd<B>(a);
+#pragma unroll
+    for (int rho = 1; rho < B; rho++) a[rho] = gl_mul(a[rho], P.twr2[(u64)rho * B + lo]);
+#pragma unroll
+    for (int rho = 0; rho < B; rho++) lds[lb + (u32)(hi * RS + brev_c<B>(rho) * (B + 1) + lo)] = a[rho];
+    LNT_SYNC();
+    // phase 3: thread (k2 = hi, kb = lo) over a
+#pragma unroll
+    for (int i = 0; i < B; i++) a[i] = lds[lb + (u32)(hi * RS + lo * (B + 1) + i)];
+    dft_fwd<B>(a);
+    if (MODE == LNT_FORWARD) {
+#pragma unroll
+        for (int i = 0; i < B; i++) P.y[base + (u64)t * B + i] = a[i];
+        return;
+    }
+    if (MODE == LNT_MULTIPLY) {
+#pragma unroll
+        for (int i = 0; i < B; i++) a[i] = gl_mul(a[i], P.f[base + (u64)t * B + i]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < B; i++) a[i] = gl_sqr(a[i]);
+    }
+    dft_inv<B>(a);
+#pragma unroll
+    for (int i = 0; i < B; i++) lds[lb + (u32)(hi * RS + lo * (B + 1) + i)] = a[i];
+    LNT_SYNC();
+    // phase 2 inverse: thread (k2, a): registers rho hold kb = brev(rho)
+#pragma unroll
+    for (int rho = 0; rho < B; rho++) {
+        u64 v = lds[lb + (u32)(hi * RS + brev_c<B>(rho) * (B + 1) + lo)];
+        a[rho] = rho ? gl_mul(v, P.itwr2[(u64)rho * B + lo]) : v;
+    }
+    dft_inv<B>(a);
+#pragma unroll
+    for (int b = 0; b < B; b++) lds[lb + (u32)(hi * RS + b * (B + 1) + lo)] = a[b];
+    LNT_SYNC();
+    // phase 1 inverse: thread i1 = t, registers rho hold k2 = brev(rho)
+#pragma unroll
+    for (int rho = 0; rho < B; rho++) {
+        u64 v = lds[lb + (u32)(brev_c<B>(rho) * RS + (t / B) * (B + 1) + t % B)];
+        a[rho] = rho ? gl_mul(v, P.itwr[(u64)rho * T + t]) : v;
+    }
+    dft_inv<B>(a);
+#pragma unroll
+    for (int j = 0; j < B; j++) P.y[base + t + (u64)T * j] = a[j];
 }
 
 // ------------------------------------------------------------------ small residues

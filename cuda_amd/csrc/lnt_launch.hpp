@@ -37,6 +37,61 @@ __global__ __launch_bounds__(G * B2) void lnt_k_row(LntParams P) {
     lnt_row_body<B1, B2, G, MODE>(threadIdx.x, blockIdx.x, LView{smem, W}, P);
 }
 
+template <int B, int G, int MODE>
+__global__ __launch_bounds__(G * B * B) void lnt_k_row3(LntParams P) {
+    constexpr u32 W = G * B * B * (B + 1);
+    __shared__ u64 smem[W];
+    lnt_row3_body<B, G, MODE>(threadIdx.x, blockIdx.x, LView{smem, W}, P);
+}
+
+// ---- batched launches: many tests of one shape in one grid.  Workgroup b serves test
+// b / wgs (its parameters Ps[b / wgs], uniform per workgroup) as that test's workgroup
+// b % wgs.  The "-2" flag is per test (Ps[].sub2).
+template <int B1, int B2, int M>
+__global__ __launch_bounds__(B2 < 64 ? 64 : B2) void lnt_kb_small(const LntParams *__restrict__ Ps, u32 iters,
+                                                                   int ll) {
+    constexpr u32 W = (B1 * B2 > B1 * (B2 + LNT_PAD)) ? B1 * B2 : B1 * (B2 + LNT_PAD);
+    __shared__ u64 smem[W];
+    const LntParams &P = Ps[blockIdx.x];
+    lnt_small_body<B1, B2, M>(threadIdx.x, LView{smem, W}, P, iters, P.sub2 != 0, ll != 0);
+}
+
+template <int B1, int B2, int K, int M>
+__global__ __launch_bounds__(K * B2) void lnt_kb_col_fwd(const LntParams *__restrict__ Ps, u32 wgs) {
+    constexpr u32 A = B1 * B2 * K, B = B1 * (B2 + LNT_PAD) * K;
+    constexpr u32 W = A > B ? A : B;
+    __shared__ u64 smem[W];
+    const LntParams &P = Ps[blockIdx.x / wgs];
+    lnt_col_fwd_body<B1, B2, K, M>(threadIdx.x, blockIdx.x % wgs, LView{smem, W}, P, P.sub2 != 0);
+}
+
+template <int B1, int B2, int K>
+__global__ __launch_bounds__(K * B2) void lnt_kb_col_inv(const LntParams *__restrict__ Ps, u32 wgs) {
+    constexpr u32 W = B1 * (B2 + LNT_PAD) * K;
+    __shared__ u64 smem[W];
+    lnt_col_inv_body<B1, B2, K>(threadIdx.x, blockIdx.x % wgs, LView{smem, W}, Ps[blockIdx.x / wgs]);
+}
+
+template <int B1, int B2, int G>
+__global__ __launch_bounds__(G * B2) void lnt_kb_row(const LntParams *__restrict__ Ps, u32 wgs) {
+    constexpr u32 W = G * B1 * (B2 + LNT_PAD);
+    __shared__ u64 smem[W];
+    lnt_row_body<B1, B2, G>(threadIdx.x, blockIdx.x % wgs, LView{smem, W}, Ps[blockIdx.x / wgs]);
+}
+
+template <int B, int G>
+__global__ __launch_bounds__(G * B * B) void lnt_kb_row3(const LntParams *__restrict__ Ps, u32 wgs) {
+    constexpr u32 W = G * B * B * (B + 1);
+    __shared__ u64 smem[W];
+    lnt_row3_body<B, G>(threadIdx.x, blockIdx.x % wgs, LView{smem, W}, Ps[blockIdx.x / wgs]);
+}
+
+// Batched launchers: `count` tests of plan pl, parameters in device memory.
+bool lnt_launch_batch_small(const lnt::Plan &pl, const LntParams *Ps, u32 count, u32 iters, bool ll, hipStream_t s);
+bool lnt_launch_batch_col_fwd(const lnt::Plan &pl, const LntParams *Ps, u32 count, hipStream_t s);
+bool lnt_launch_batch_col_inv(const lnt::Plan &pl, const LntParams *Ps, u32 count, hipStream_t s);
+bool lnt_launch_batch_row(const lnt::Plan &pl, const LntParams *Ps, u32 count, hipStream_t s);
+
 // Launchers (defined in lnt_k_small.hip / lnt_k_col.hip / lnt_k_row.hip).
 // mode: LNT_SQUARE (iters squarings), LNT_FORWARD / LNT_MULTIPLY (one pass, iters = 1)
 bool lnt_launch_small(const lnt::Plan &pl, const LntParams &P, u32 iters, bool sub2_first, bool ll,

@@ -31,7 +31,9 @@ EXPORTS = (
     "lnt_create", "lnt_set_digits", "lnt_iterate", "lnt_get_digits", "lnt_res64",
     "lnt_sync", "lnt_tune", "lnt_destroy", "lnt_last_error", "lnt_secured_log2_length",
     "lnt_length", "lnt_describe", "lnt_profile", "lnt_mul", "lnt_copy", "lnt_inject_error",
-    "lnt_gerbicz_begin", "lnt_gerbicz_iterate", "lnt_gerbicz_stats",
+    "lnt_gerbicz_begin", "lnt_gerbicz_iterate", "lnt_gerbicz_stats", "lnt_batch_create",
+    "lnt_batch_add", "lnt_batch_remove", "lnt_batch_size", "lnt_batch_iterate", "lnt_batch_sync",
+    "lnt_batch_destroy", "lnt_batch_last_error",
 )
 
 
@@ -81,6 +83,16 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lnt_gerbicz_begin.argtypes = [vp, u32, u32]
     lib.lnt_gerbicz_iterate.argtypes = [vp, u32]
     lib.lnt_gerbicz_stats.argtypes = [vp, u64p, ctypes.c_int]
+    lib.lnt_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    lib.lnt_batch_add.argtypes = [vp, vp]
+    lib.lnt_batch_remove.argtypes = [vp, vp]
+    lib.lnt_batch_size.argtypes = [vp]
+    lib.lnt_batch_iterate.argtypes = [vp, u32]
+    lib.lnt_batch_sync.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    lib.lnt_batch_destroy.argtypes = [vp]
+    lib.lnt_batch_destroy.restype = None
+    lib.lnt_batch_last_error.argtypes = [vp]
+    lib.lnt_batch_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
 
@@ -234,3 +246,52 @@ class LucasPRP:
             raise LntError(LNT_DEVICE_ERROR, self._lib.lnt_last_error(self._h).decode())
         self.iteration += int(iters)
         return list(zip(names.value.decode().split(";"), [float(ms[i]) for i in range(cnt)]))
+
+
+class Batch:
+    """Independent tests advanced together: one hipGraph launch per iteration for all
+    members (members of one shape share batched kernel launches)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load()
+        h = ctypes.c_void_p()
+        rc = self._lib.lnt_batch_create(device, ctypes.byref(h))
+        if rc != LNT_OK:
+            raise LntError(rc, self._lib.lnt_last_error(None).decode())
+        self._h = h
+        self.members: List[LucasPRP] = []
+
+    def _check(self, rc: int) -> None:
+        if rc != LNT_OK:
+            raise LntError(rc, self._lib.lnt_batch_last_error(self._h).decode())
+
+    def add(self, t: LucasPRP) -> None:
+        self._check(self._lib.lnt_batch_add(self._h, t._h))
+        self.members.append(t)
+
+    def remove(self, t: LucasPRP) -> None:
+        self._check(self._lib.lnt_batch_remove(self._h, t._h))
+        self.members.remove(t)
+
+    def iterate(self, k: int) -> None:
+        """Every member advances k iterations (asynchronous; call sync() before using a
+        member on its own)."""
+        self._check(self._lib.lnt_batch_iterate(self._h, int(k)))
+        for t in self.members:
+            t.iteration += int(k)
+
+    def sync(self) -> float:
+        ms = ctypes.c_float()
+        self._check(self._lib.lnt_batch_sync(self._h, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.lnt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -113,6 +113,22 @@ int lnt_gerbicz_iterate(lnt_ctx *ctx, uint32_t k);
  * [3] rollbacks, [4] res64 of D at the last verified checkpoint.  cap >= 5. */
 int lnt_gerbicz_stats(lnt_ctx *ctx, uint64_t *out, int cap);
 
+/* ---- batches of independent tests (new: the reference runs one test per process) ----
+ * Members keep their own residues; members of the same shape run as one batched launch
+ * per kernel and one iteration of all members is a single hipGraph launch.  Between
+ * lnt_batch_iterate and lnt_batch_sync a member must not be used on its own. */
+typedef struct lnt_batch lnt_batch;
+int lnt_batch_create(int device, lnt_batch **out);
+int lnt_batch_add(lnt_batch *batch, lnt_ctx *ctx);
+int lnt_batch_remove(lnt_batch *batch, lnt_ctx *ctx);
+int lnt_batch_size(const lnt_batch *batch);
+/* Every member advances k iterations (asynchronous). */
+int lnt_batch_iterate(lnt_batch *batch, uint32_t k);
+/* Wait; *elapsed_ms = GPU time since the previous lnt_batch_sync (or create). */
+int lnt_batch_sync(lnt_batch *batch, float *elapsed_ms);
+void lnt_batch_destroy(lnt_batch *batch);
+const char *lnt_batch_last_error(const lnt_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

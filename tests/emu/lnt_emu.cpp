@@ -21,7 +21,7 @@ struct Emu {
     Plan pl;
     uint32_t p;
     bool prp, pending;
-    std::vector<uint64_t> x, y, f, w, uw, tw4, itw4, twr, itwr, twc, itwc;
+    std::vector<uint64_t> x, y, f, w, uw, tw4, itw4, twr, itwr, twr2, itwr2, twc, itwc;
     std::vector<uint8_t> wid;
     LntParams P;
 };
@@ -41,11 +41,15 @@ static void line_tables(uint32_t L, Line ln, std::vector<uint64_t> &tw, std::vec
     }
 }
 
-extern "C" void *emu_create(uint32_t p, uint32_t n, int prp, uint32_t force_lc) {
+extern "C" void *emu_create2(uint32_t p, uint32_t n, int prp, uint32_t lc, uint32_t K, int row3) {
     Emu *e = new Emu;
     e->p = p;
     e->prp = prp;
-    std::string err = make_plan(p, n, e->pl, force_lc);
+    PlanOpts o;
+    o.lc = lc;
+    o.K = K;
+    o.row3 = row3;
+    std::string err = make_plan(p, n, e->pl, o);
     if (!err.empty()) {
         std::fprintf(stderr, "emu_create: %s\n", err.c_str());
         delete e;
@@ -64,7 +68,18 @@ extern "C" void *emu_create(uint32_t p, uint32_t n, int prp, uint32_t force_lc) 
         e->uw[i] = gl_mul(gl_inv(e->w[i]), ninv);
     }
     widths(p, n, e->wid);
-    line_tables(pl.small ? n : pl.C, pl.row, e->twr, e->itwr);
+    if (pl.row3) {
+        Line l1, l2;
+        l1.B1 = pl.row3;
+        l1.B2 = pl.row3 * pl.row3;
+        l2.B1 = l2.B2 = pl.row3;
+        line_tables(pl.C, l1, e->twr, e->itwr);
+        line_tables(pl.row3 * pl.row3, l2, e->twr2, e->itwr2);
+    } else {
+        line_tables(pl.small ? n : pl.C, pl.row, e->twr, e->itwr);
+        e->twr2.assign(1, 1);
+        e->itwr2.assign(1, 1);
+    }
     if (!pl.small) {
         line_tables(pl.R, pl.col, e->twc, e->itwc);
         e->tw4.resize(n);
@@ -89,6 +104,8 @@ extern "C" void *emu_create(uint32_t p, uint32_t n, int prp, uint32_t force_lc) 
     P.itw4 = gv<const u64>(e->itw4);
     P.twr = gv<const u64>(e->twr);
     P.itwr = gv<const u64>(e->itwr);
+    P.twr2 = gv<const u64>(e->twr2);
+    P.itwr2 = gv<const u64>(e->itwr2);
     P.twc = gv<const u64>(e->twc);
     P.itwc = gv<const u64>(e->itwc);
     P.p = p;
@@ -102,6 +119,10 @@ extern "C" void *emu_create(uint32_t p, uint32_t n, int prp, uint32_t force_lc) 
     to_device_digits(e->wid, d.data(), e->x.data());
     e->pending = false;
     return e;
+}
+
+extern "C" void *emu_create(uint32_t p, uint32_t n, int prp, uint32_t force_lc) {
+    return emu_create2(p, n, prp, force_lc, 0, -1);
 }
 
 extern "C" void emu_destroy(void *h) { delete (Emu *)h; }
@@ -137,6 +158,18 @@ static void col_inv(Emu *e, const LntParams &P) {
     uint32_t lds = B1 * (B2 + LNT_PAD) * K;
     for (uint32_t b = 0; b < pl.C / K; b++)
         run_wg(K * B2, lds, [&](int tid, LView L) { lnt_col_inv_body<B1, B2, K>(tid, b, L, P); });
+}
+
+template <int B, int G>
+static void row3_sq(Emu *e, const LntParams &P, int mode) {
+    const Plan &pl = e->pl;
+    uint32_t lds = G * B * B * (B + 1);
+    for (uint32_t b = 0; b < pl.R / G; b++)
+        run_wg(G * B * B, lds, [&](int tid, LView L) {
+            if (mode == LNT_FORWARD) lnt_row3_body<B, G, LNT_FORWARD>(tid, b, L, P);
+            else if (mode == LNT_MULTIPLY) lnt_row3_body<B, G, LNT_MULTIPLY>(tid, b, L, P);
+            else lnt_row3_body<B, G>(tid, b, L, P);
+        });
 }
 
 template <int B1, int B2, int G>
@@ -181,10 +214,13 @@ static bool run_large_once(Emu *e, bool sub2, const LntParams &P, int mode = LNT
         if (pl.M == 4) col_fwd<b1, b2, k, 4>(e, sub2, P); else col_fwd<b1, b2, k, 8>(e, sub2, P); \
         ok = true;                                                               \
     }
-    CF(8, 8, 16) CF(16, 8, 16) CF(16, 16, 16) CF(32, 16, 16) CF(32, 32, 16) CF(64, 32, 8) CF(64, 64, 4)
+    CF(8, 8, 16) CF(16, 8, 16) CF(16, 16, 16) CF(32, 16, 16) CF(32, 32, 16) CF(32, 32, 8) CF(64, 32, 8) CF(64, 32, 4)
+    CF(64, 64, 4)
 #undef CF
     if (!ok) return false;
     ok = false;
+    if (pl.row3 == 16) row3_sq<16, 1>(e, P, mode), ok = true;
+    if (pl.row3 == 8) row3_sq<8, 4>(e, P, mode), ok = true;
 #define RW(b1, b2)                                                        \
     if (!ok && pl.row.B1 == b1 && pl.row.B2 == b2) {                      \
         row_sq<b1, b2, 64 / b2>(e, P, mode);                              \
@@ -200,7 +236,8 @@ static bool run_large_once(Emu *e, bool sub2, const LntParams &P, int mode = LNT
         col_inv<b1, b2, k>(e, P);                                               \
         ok = true;                                                              \
     }
-    CI(8, 8, 16) CI(16, 8, 16) CI(16, 16, 16) CI(32, 16, 16) CI(32, 32, 16) CI(64, 32, 8) CI(64, 64, 4)
+    CI(8, 8, 16) CI(16, 8, 16) CI(16, 16, 16) CI(32, 16, 16) CI(32, 32, 16) CI(32, 32, 8) CI(64, 32, 8) CI(64, 32, 4)
+    CI(64, 64, 4)
 #undef CI
     return ok;
 }

@@ -100,8 +100,10 @@ def emu():
         if not os.path.exists(EMU_SO) or any(os.path.getmtime(d) > os.path.getmtime(EMU_SO)
                                              for d in deps):
             os.makedirs(os.path.dirname(EMU_SO), exist_ok=True)
+            tmp = f"{EMU_SO}.{os.getpid()}.tmp"  # parallel test workers may build at once
             subprocess.run(["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-o",
-                            EMU_SO, EMU_SRC], check=True)
+                            tmp, EMU_SRC], check=True)
+            os.replace(tmp, EMU_SO)
         L = ctypes.CDLL(EMU_SO)
         u32, vp = ctypes.c_uint32, ctypes.c_void_p
         L.emu_create.argtypes = [u32, u32, ctypes.c_int, u32]
