@@ -83,6 +83,9 @@ def main() -> int:
     ap.add_argument("--profile-iters", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain copy-like kernel (int64 negate) over MIB MiB, three "
+                         "times: the reference point that scales HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the hot path is a single test on one device "
@@ -92,6 +95,18 @@ def main() -> int:
 
     import cuda_amd
     from cuda_amd import LucasPRP
+
+    if args.copy_calibration:
+        import torch
+        words = args.copy_calibration * (1 << 20) // 8
+        a = torch.arange(words, dtype=torch.int64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        log(f"bench: copy calibration: 3 x int64 negate of {args.copy_calibration} MiB "
+            f"(reads {8 * words} B, writes {8 * words} B per launch)")
+        del a, b
 
     p = next_prime(70_000_000 + args.seed % 100_000)
     t = LucasPRP(p)

@@ -103,40 +103,82 @@ constexpr int LNT_PAD = 1;
 
 // ------------------------------------------------------------------ column forward
 // grid: C / K workgroups (column tile c0 = bid * K), block: K * B2 threads.
-// LDS words: max(R * K, B1 * (B2 + PAD) * K).
+// LDS words: lnt_col_fwd_lds<B1, B2, K>().
+// Row stride of the carry staging layout: one word of padding keeps the per-row carry
+// accesses (one thread per row) on distinct LDS banks while the stage fits in 144 KiB.
+template <int B1, int B2, int K>
+constexpr int lnt_col_ks() { return B1 * B2 * (K + 1) <= 18432 ? K + 1 : K; }
+template <int B1, int B2, int K>
+constexpr u32 lnt_col_fwd_lds() {
+    return (u32)(B1 * B2 * lnt_col_ks<B1, B2, K>() > B1 * (B2 + LNT_PAD) * K ? B1 * B2 * lnt_col_ks<B1, B2, K>()
+                                                                           : B1 * (B2 + LNT_PAD) * K);
+}
+
 template <int B1, int B2, int K, int M>
 LNT_KFN void lnt_col_fwd_body(int tid, int bid, LView lds, const LntParams &P, bool sub2) {
     constexpr int T = K * B2;
     constexpr int RR = B1 * B2;
     constexpr int RP = B1 / B2;
     constexpr int S = B2 + LNT_PAD;
+    constexpr int KS = lnt_col_ks<B1, B2, K>();
+    constexpr int RT = (RR + T - 1) / T;  // carry rows per thread (the last may be idle)
     const u64 C = P.C;
     const u64 n = P.n;
     const u64 c0 = (u64)bid * K;
+    const int cl = tid % K, t = tid / K;
+    const u64 c = c0 + cl;
 
-    // 1. carry + (-2): one thread per row run of K digits (plus M halo digits)
-    for (int r = tid; r < RR; r += T) {
-        int64_t v[M + K];
-        const u64 g0 = C * r + c0;
+    // 0. loads: the tile (K adjacent lanes read one row run: 8K contiguous bytes), the M
+    //    digits before each of this thread's carry rows, and the weights of phase 1.
+    // Tables are prefetched into registers when a thread's line fits (B1 <= 32).
+    // The halo digits are prefetched too when there are few of them per thread.
+    constexpr bool PF = B1 <= 32, HP = RT * M <= 16;
+    u64 a[B1], wv[PF ? B1 : 1], h[HP ? RT : 1][M];
 #pragma unroll
-        for (int j = 0; j < M + K; j++) {
-            u64 g = (g0 + n - M + j) & (n - 1);
-            v[j] = gl_to_i64(P.x[g]);
+    for (int j = 0; j < B1; j++) a[j] = P.x[c + C * (u64)(t + B2 * j)];
+    if (HP) {
+#pragma unroll
+        for (int q = 0; q < RT; q++) {
+            const u64 g0 = C * (u64)(tid + T * q) + c0;
+#pragma unroll
+            for (int j = 0; j < M; j++)
+                h[HP ? q : 0][j] = (tid + T * q < RR) ? P.x[(g0 + n - M + j) & (n - 1)] : 0;
         }
+    }
+    if (PF) {
+#pragma unroll
+        for (int j = 0; j < B1; j++) wv[PF ? j : 0] = P.w[c + C * (u64)(t + B2 * j)];
+    }
+#pragma unroll
+    for (int j = 0; j < B1; j++) lds[(u32)((t + B2 * j) * KS + cl)] = a[j];
+    LNT_SYNC();
+
+    // 1. carry + (-2): one thread per row run of K digits (plus the M halo digits)
+#pragma unroll
+    for (int q = 0; q < RT; q++) {
+        const int r = tid + T * q;
+        if (r >= RR) break;
+        int64_t v[M + K];
+        const u64 g0 = C * (u64)r + c0;
+#pragma unroll
+        for (int j = 0; j < M; j++) v[j] = gl_to_i64(HP ? h[HP ? q : 0][j] : P.x[(g0 + n - M + j) & (n - 1)]);
+#pragma unroll
+        for (int j = 0; j < K; j++) v[M + j] = gl_to_i64(lds[(u32)(r * KS + j)]);
         lnt_carry_run<M, K>(v, P.p, P.log_n, g0, P.levels, sub2);
 #pragma unroll
-        for (int c = 0; c < K; c++) lds[(u32)(r * K + c)] = gl_from_i64(v[M + c]);
+        for (int j = 0; j < K; j++) lds[(u32)(r * KS + j)] = gl_from_i64(v[M + j]);
     }
     LNT_SYNC();
 
-    // 2. phase 1: weight, B1-point DFT down the column, inter-block twiddle
-    const int cl = tid % K, t = tid / K;
-    const u64 c = c0 + cl;
-    u64 a[B1];
+    // 2. phase 1: weight, B1-point DFT down the column, inter-block twiddle.  The 4-step
+    //    twiddles of phase 3 are fetched here so their latency hides behind the DFTs.
 #pragma unroll
-    for (int j = 0; j < B1; j++) {
-        const u64 r = t + B2 * j;
-        a[j] = gl_mul(lds[(u32)(r * K + cl)], P.w[c + C * r]);
+    for (int j = 0; j < B1; j++)
+        a[j] = gl_mul(lds[(u32)((t + B2 * j) * KS + cl)], PF ? wv[PF ? j : 0] : P.w[c + C * (u64)(t + B2 * j)]);
+    u64 tv[PF ? B1 : 1];
+    if (PF) {
+#pragma unroll
+        for (int i = 0; i < B1; i++) tv[PF ? i : 0] = P.tw4[c + C * ((u64)t * B1 + i)];
     }
     dft_fwd<B1>(a);
     if (B2 > 1) {
@@ -160,7 +202,7 @@ LNT_KFN void lnt_col_fwd_body(int tid, int bid, LView lds, const LntParams &P, b
 #pragma unroll
     for (int i = 0; i < B1; i++) {
         const u64 idx = c + C * ((u64)t * B1 + i);
-        P.y[idx] = gl_mul(a[i], P.tw4[idx]);
+        P.y[idx] = gl_mul(a[i], PF ? tv[PF ? i : 0] : P.tw4[idx]);
     }
 }
 
@@ -172,11 +214,16 @@ LNT_KFN void lnt_col_inv_body(int tid, int bid, LView lds, const LntParams &P) {
     const u64 C = P.C;
     const int cl = tid % K, t = tid / K;
     const u64 c = (u64)bid * K + cl;
-    u64 a[B1];
+    constexpr bool PF = B1 <= 32;  // prefetch the unweighting factors (fits when B1 <= 32)
+    u64 a[B1], uv[PF ? B1 : 1];
 #pragma unroll
     for (int i = 0; i < B1; i++) {
         const u64 idx = c + C * ((u64)t * B1 + i);
         a[i] = gl_mul(P.y[idx], P.itw4[idx]);
+    }
+    if (PF) {
+#pragma unroll
+        for (int j = 0; j < B1; j++) uv[PF ? j : 0] = P.uw[c + C * ((u64)t + (u64)B2 * j)];
     }
 #pragma unroll
     for (int q = 0; q < RP; q++) {
@@ -195,7 +242,7 @@ LNT_KFN void lnt_col_inv_body(int tid, int bid, LView lds, const LntParams &P) {
 #pragma unroll
     for (int j = 0; j < B1; j++) {
         const u64 idx = c + C * ((u64)t + (u64)B2 * j);
-        P.x[idx] = gl_mul(a[j], P.uw[idx]);
+        P.x[idx] = gl_mul(a[j], PF ? uv[PF ? j : 0] : P.uw[idx]);
     }
 }
 

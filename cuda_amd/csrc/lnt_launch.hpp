@@ -8,6 +8,14 @@
 #include "lnt_host.hpp"
 #include "lnt_kernels.hpp"
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup i to XCD i % 8), each
+// XCD with its own L2.  Renumbering them so that each XCD gets a contiguous range of
+// tiles lets a column tile find its left neighbour's boundary lines (its carry halo) in
+// the same L2.
+__device__ __forceinline__ u32 lnt_xcd_bid(u32 b, u32 grid) {
+    return (grid & 7) ? b : (b & 7) * (grid >> 3) + (b >> 3);
+}
+
 template <int B1, int B2, int M, int MODE>
 __global__ __launch_bounds__(B2 < 64 ? 64 : B2) void lnt_k_small(LntParams P, u32 iters, int sub2_first, int ll) {
     constexpr u32 W = (B1 * B2 > B1 * (B2 + LNT_PAD)) ? B1 * B2 : B1 * (B2 + LNT_PAD);
@@ -17,17 +25,16 @@ __global__ __launch_bounds__(B2 < 64 ? 64 : B2) void lnt_k_small(LntParams P, u3
 
 template <int B1, int B2, int K, int M>
 __global__ __launch_bounds__(K * B2) void lnt_k_col_fwd(LntParams P, int sub2) {
-    constexpr u32 A = B1 * B2 * K, B = B1 * (B2 + LNT_PAD) * K;
-    constexpr u32 W = A > B ? A : B;
+    constexpr u32 W = lnt_col_fwd_lds<B1, B2, K>();
     __shared__ u64 smem[W];
-    lnt_col_fwd_body<B1, B2, K, M>(threadIdx.x, blockIdx.x, LView{smem, W}, P, sub2 != 0);
+    lnt_col_fwd_body<B1, B2, K, M>(threadIdx.x, lnt_xcd_bid(blockIdx.x, gridDim.x), LView{smem, W}, P, sub2 != 0);
 }
 
 template <int B1, int B2, int K>
 __global__ __launch_bounds__(K * B2) void lnt_k_col_inv(LntParams P) {
     constexpr u32 W = B1 * (B2 + LNT_PAD) * K;
     __shared__ u64 smem[W];
-    lnt_col_inv_body<B1, B2, K>(threadIdx.x, blockIdx.x, LView{smem, W}, P);
+    lnt_col_inv_body<B1, B2, K>(threadIdx.x, lnt_xcd_bid(blockIdx.x, gridDim.x), LView{smem, W}, P);
 }
 
 template <int B1, int B2, int G, int MODE>
@@ -58,18 +65,19 @@ __global__ __launch_bounds__(B2 < 64 ? 64 : B2) void lnt_kb_small(const LntParam
 
 template <int B1, int B2, int K, int M>
 __global__ __launch_bounds__(K * B2) void lnt_kb_col_fwd(const LntParams *__restrict__ Ps, u32 wgs) {
-    constexpr u32 A = B1 * B2 * K, B = B1 * (B2 + LNT_PAD) * K;
-    constexpr u32 W = A > B ? A : B;
+    constexpr u32 W = lnt_col_fwd_lds<B1, B2, K>();
     __shared__ u64 smem[W];
-    const LntParams &P = Ps[blockIdx.x / wgs];
-    lnt_col_fwd_body<B1, B2, K, M>(threadIdx.x, blockIdx.x % wgs, LView{smem, W}, P, P.sub2 != 0);
+    const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
+    const LntParams &P = Ps[b / wgs];
+    lnt_col_fwd_body<B1, B2, K, M>(threadIdx.x, b % wgs, LView{smem, W}, P, P.sub2 != 0);
 }
 
 template <int B1, int B2, int K>
 __global__ __launch_bounds__(K * B2) void lnt_kb_col_inv(const LntParams *__restrict__ Ps, u32 wgs) {
     constexpr u32 W = B1 * (B2 + LNT_PAD) * K;
     __shared__ u64 smem[W];
-    lnt_col_inv_body<B1, B2, K>(threadIdx.x, blockIdx.x % wgs, LView{smem, W}, Ps[blockIdx.x / wgs]);
+    const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
+    lnt_col_inv_body<B1, B2, K>(threadIdx.x, b % wgs, LView{smem, W}, Ps[b / wgs]);
 }
 
 template <int B1, int B2, int G>

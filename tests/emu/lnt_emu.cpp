@@ -147,7 +147,7 @@ static void small_iter(Emu *e, uint32_t k, const LntParams &P, bool sub2) {
 template <int B1, int B2, int K, int M>
 static void col_fwd(Emu *e, bool sub2, const LntParams &P) {
     const Plan &pl = e->pl;
-    uint32_t lds = std::max<uint32_t>(pl.R * K, B1 * (B2 + LNT_PAD) * K);
+    uint32_t lds = lnt_col_fwd_lds<B1, B2, K>();
     for (uint32_t b = 0; b < pl.C / K; b++)
         run_wg(K * B2, lds, [&](int tid, LView L) { lnt_col_fwd_body<B1, B2, K, M>(tid, b, L, P, sub2); });
 }
