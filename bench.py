@@ -74,6 +74,68 @@ def cpu_baseline(p: int, n: int, seconds: float):
                       f"iterations ({dt:.1f} s), oracle/lnt_oracle.c, CPU: {cpu}"}
 
 
+def batched_leg(seed: int, iters: int, chunk: int = 1000):
+    """The batched uneven workload: 64 LL tests, exponents log-uniform in [1M, 20M] from
+    splitmix64(seed), 10,000 iterations each; all advance together (one hipGraph launch per
+    iteration) versus the same tests one at a time on the plain path.  Both timings include
+    each test's set-up (tables) and its final res64 readout."""
+    from cuda_amd import Batch, LucasPRP
+    from cuda_amd.workloads import batched_exponents
+    ps = batched_exponents(seed)
+    t0 = time.perf_counter()
+    batch = Batch()
+    ts = [LucasPRP(p) for p in ps]
+    for t in ts:
+        batch.add(t)
+    done = 0
+    while done < iters:
+        k = min(chunk, iters - done)
+        batch.iterate(k)
+        done += k
+    batch.sync()
+    res_b = [t.res64() for t in ts]
+    tb = time.perf_counter() - t0
+    batch.close()
+    for t in ts:
+        t.close()
+    log(f"bench: batched {len(ps)} tests x {iters} iterations in {tb:.2f} s")
+    t0 = time.perf_counter()
+    res_p = []
+    for p in ps:
+        t = LucasPRP(p)
+        t.iterate(iters)
+        res_p.append(t.res64())
+        t.close()
+    tp = time.perf_counter() - t0
+    log(f"bench: one at a time {tp:.2f} s")
+    gmp_checked = gmp_equal = 0
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from helpers import golden
+        for p, r in zip(ps, res_b):
+            g = golden(p, "LL") or {}
+            if iters in g:
+                gmp_checked += 1
+                gmp_equal += int(g[iters] == r)
+    except Exception as e:  # fixtures are optional for the measurement
+        log("bench: no fixture check:", e)
+    return {"workload": f"{len(ps)} LL tests, p log-uniform in [1e6, 2e7] (splitmix64 seed {seed}), "
+                        f"n = 2^{min(t for t in map(secured, ps))}..2^{max(map(secured, ps))}, "
+                        f"{iters} iterations each",
+            "tests": len(ps), "iterations": iters,
+            "batched_s": round(tb, 3), "plain_s": round(tp, 3),
+            "tests_per_hour": round(len(ps) / tb * 3600, 1),
+            "plain_tests_per_hour": round(len(ps) / tp * 3600, 1),
+            "speedup": round(tp / tb, 3),
+            "res64_equal_to_plain": res_b == res_p,
+            "res64_checked_against_gmp": gmp_checked, "res64_equal_to_gmp": gmp_equal}
+
+
+def secured(p: int) -> int:
+    from cuda_amd import secured_log2_length
+    return secured_log2_length(p)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +145,8 @@ def main() -> int:
     ap.add_argument("--profile-iters", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-iters", type=int, default=10000,
+                    help="iterations per test of the batched workload (0: skip that leg)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain copy-like kernel (int64 negate) over MIB MiB, three "
                          "times: the reference point that scales HBM byte counters")
@@ -174,6 +238,8 @@ def main() -> int:
                                    "achieved": round(32 * n / (iter_ms * 1e-3) / 1e9, 1),
                                    "frac": round(32 * n / (iter_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
     }
+    if args.batch_iters:
+        line["batched"] = batched_leg(args.seed, args.batch_iters)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p, n, args.cpu_seconds)
     print(json.dumps(line), flush=True)

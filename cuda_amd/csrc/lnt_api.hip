@@ -47,9 +47,11 @@ __global__ void lnt_k_tw4(u64 om, u32 C, u32 B1, u32 B2, u32 n, u64 *tw) {
 // ------------------------------------------------------------------ context
 
 struct lnt_ctx {
+    struct lnt_batch *batch = nullptr;  // the batch this context belongs to, if any
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev_mark = nullptr;
+    hipEvent_t ev_gz = nullptr;  // end of a Gerbicz block step (a batch waits on it)
     Plan pl;
     uint32_t p = 0;
     bool prp = false;
@@ -337,6 +339,7 @@ static int make_context(int device, uint32_t p, uint64_t n, bool prp, const Plan
         std::string m = c->err;
         free_buffers(c);
         if (c->ev_mark) (void)hipEventDestroy(c->ev_mark);
+        if (c->ev_gz) (void)hipEventDestroy(c->ev_gz);
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return fail(nullptr, code, m);
@@ -347,7 +350,7 @@ static int make_context(int device, uint32_t p, uint64_t n, bool prp, const Plan
         return bail(LNT_DEVICE_ERROR);
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev_mark) != hipSuccess) {
+        hipEventCreate(&c->ev_mark) != hipSuccess || hipEventCreateWithFlags(&c->ev_gz, hipEventDisableTiming) != hipSuccess) {
         c->err = "hipStreamCreate / hipEventCreate failed";
         return bail(LNT_DEVICE_ERROR);
     }
@@ -450,10 +453,12 @@ int lnt_sync(lnt_ctx *c, float *elapsed_ms) {
 
 void lnt_destroy(lnt_ctx *c) {
     if (!c) return;
+    if (c->batch) (void)lnt_batch_remove(c->batch, c);
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_buffers(c);
     if (c->ev_mark) (void)hipEventDestroy(c->ev_mark);
+    if (c->ev_gz) (void)hipEventDestroy(c->ev_gz);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -711,6 +716,38 @@ static int gz_check(lnt_ctx *c, bool &passed) {
     return LNT_OK;
 }
 
+// The step after the squarings that end a block (g.it % B == 0): D_prev <- D, D <- D * u;
+// every L blocks the check, then save the verified checkpoint or restore it (g.it falls
+// back to the checkpoint's iteration).
+static int gz_block_end(lnt_ctx *c) {
+    auto &g = c->g;
+    int rc;
+    if ((rc = copy_buf(c, g.Dprev.x, g.D.x))) return rc;
+    g.Dprev.pending = g.D.pending;
+    if ((rc = mul_buf(c, g.D.x, g.D.pending, c->x, c->pending))) return rc;
+    if ((g.it / g.B) % g.L) return LNT_OK;
+    bool passed = false;
+    if ((rc = gz_check(c, passed))) return rc;
+    if (passed) {
+        g.checks++;
+        g.fails_here = 0;
+        g.it_v = g.it;
+        if ((rc = copy_buf(c, g.Uv.x, c->x))) return rc;
+        g.Uv.pending = c->pending;
+        if ((rc = copy_buf(c, g.Dv.x, g.D.x))) return rc;
+        g.Dv.pending = g.D.pending;
+    } else {
+        g.rollbacks++;
+        if (++g.fails_here > 3) return fail(c, LNT_DEVICE_ERROR, "Gerbicz check failed 4 times from the same checkpoint");
+        if ((rc = copy_buf(c, c->x, g.Uv.x))) return rc;
+        c->pending = g.Uv.pending;
+        if ((rc = copy_buf(c, g.D.x, g.Dv.x))) return rc;
+        g.D.pending = g.Dv.pending;
+        g.it = g.it_v;
+    }
+    return LNT_OK;
+}
+
 int lnt_gerbicz_begin(lnt_ctx *c, uint32_t block, uint32_t blocks_per_check) {
     if (!c) return fail(c, LNT_INVALID_ARGUMENT, "NULL context");
     if (!c->prp) return fail(c, LNT_INVALID_ARGUMENT, "the Gerbicz check applies to PRP tests");
@@ -746,32 +783,7 @@ int lnt_gerbicz_iterate(lnt_ctx *c, uint32_t k) {
         if ((rc = square_buf(c, c->x, c->pending, (uint32_t)step))) return rc;
         c->pristine = false;
         g.it += step;
-        if (g.it % g.B) continue;
-        // block boundary: D_prev <- D, D <- D * u
-        if ((rc = copy_buf(c, g.Dprev.x, g.D.x))) return rc;
-        g.Dprev.pending = g.D.pending;
-        if ((rc = mul_buf(c, g.D.x, g.D.pending, c->x, c->pending))) return rc;
-        if ((g.it / g.B) % g.L) continue;
-        bool passed = false;
-        if ((rc = gz_check(c, passed))) return rc;
-        if (passed) {
-            g.checks++;
-            g.fails_here = 0;
-            g.it_v = g.it;
-            if ((rc = copy_buf(c, g.Uv.x, c->x))) return rc;
-            g.Uv.pending = c->pending;
-            if ((rc = copy_buf(c, g.Dv.x, g.D.x))) return rc;
-            g.Dv.pending = g.D.pending;
-        } else {
-            g.rollbacks++;
-            if (++g.fails_here > 3)
-                return fail(c, LNT_DEVICE_ERROR, "Gerbicz check failed 4 times from the same checkpoint");
-            if ((rc = copy_buf(c, c->x, g.Uv.x))) return rc;
-            c->pending = g.Uv.pending;
-            if ((rc = copy_buf(c, g.D.x, g.Dv.x))) return rc;
-            g.D.pending = g.Dv.pending;
-            g.it = g.it_v;
-        }
+        if (g.it % g.B == 0 && (rc = gz_block_end(c))) return rc;
     }
     return LNT_OK;
 }
@@ -811,6 +823,7 @@ struct lnt_batch {
     };
     std::vector<Group> groups;
     hipEvent_t fork = nullptr;
+    hipEvent_t done = nullptr;  // end of the last batch_iterate: members' streams wait on it
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool dirty = true;
@@ -928,6 +941,7 @@ int lnt_batch_create(int device, lnt_batch **out) {
     b->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev_mark) != hipSuccess || hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
         hipEventRecord(b->ev_mark, b->stream) != hipSuccess) {
         delete b;
         return fail(nullptr, LNT_DEVICE_ERROR, "lnt_batch_create: HIP stream/event set-up failed");
@@ -939,11 +953,11 @@ int lnt_batch_create(int device, lnt_batch **out) {
 int lnt_batch_add(lnt_batch *b, lnt_ctx *c) {
     if (!b || !c) return bfail(b, LNT_INVALID_ARGUMENT, "NULL argument");
     if (c->device != b->device) return bfail(b, LNT_INVALID_ARGUMENT, "context is on another device");
-    for (lnt_ctx *m : b->members)
-        if (m == c) return bfail(b, LNT_INVALID_ARGUMENT, "context is already in the batch");
+    if (c->batch) return bfail(b, LNT_INVALID_ARGUMENT, "context is already in a batch");
     BHIPCHK(b, hipSetDevice(b->device), "hipSetDevice");
     BHIPCHK(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
     b->members.push_back(c);
+    c->batch = b;
     b->dirty = true;
     return LNT_OK;
 }
@@ -955,6 +969,7 @@ int lnt_batch_remove(lnt_batch *b, lnt_ctx *c) {
     for (size_t i = 0; i < b->members.size(); i++)
         if (b->members[i] == c) {
             b->members.erase(b->members.begin() + i);
+            c->batch = nullptr;
             b->dirty = true;
             return LNT_OK;
         }
@@ -973,9 +988,43 @@ int lnt_batch_iterate(lnt_batch *b, uint32_t k) {
         for (size_t i = 0; i < g.ctxs.size() && !b->dirty; i++)
             if (g.h[i].x.p != g.ctxs[i]->P.x.p || g.h[i].twr.p != g.ctxs[i]->P.twr.p) b->dirty = true;
     if (b->dirty && (rc = batch_build(b))) return rc;
-    for (uint32_t i = 0; i < k; i++) {
-        BHIPCHK(b, hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch");
-        if (i == 0 && (rc = batch_mark_pending(b))) return rc;
+    // Members under a Gerbicz check count the squarings as checked iterations: the
+    // replays stop at each of their block ends for the block step on the member's stream.
+    // A batch of single-workgroup tests only runs `step` iterations in one launch per
+    // group (the kernel loops over iterations), without the graph.
+    bool all_small = true;
+    for (auto &g : b->groups) all_small = all_small && g.pl.small;
+    uint32_t left = k;
+    bool first = true;
+    while (left) {
+        uint32_t step = left;
+        for (lnt_ctx *c : b->members)
+            if (c->g.on) step = (uint32_t)std::min<uint64_t>(step, c->g.B - c->g.it % c->g.B);
+        if (all_small) {
+            for (auto &g : b->groups)
+                if (!lnt_launch_batch_small(g.pl, g.d, (u32)g.ctxs.size(), step, g.ll, b->stream))
+                    return bfail(b, LNT_UNSUPPORTED, "no batched kernel for this shape");
+            BHIPCHK(b, hipGetLastError(), "kernel launch");
+            if (first && (rc = batch_mark_pending(b))) return rc;
+            first = false;
+        }
+        for (uint32_t i = 0; i < step && !all_small; i++) {
+            BHIPCHK(b, hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch");
+            if (first && (rc = batch_mark_pending(b))) return rc;
+            first = false;
+        }
+        left -= step;
+        // later work on a member's own stream starts after the batch's
+        BHIPCHK(b, hipEventRecord(b->done, b->stream), "hipEventRecord");
+        for (lnt_ctx *c : b->members) BHIPCHK(b, hipStreamWaitEvent(c->stream, b->done, 0), "hipStreamWaitEvent");
+        for (lnt_ctx *c : b->members) {
+            if (!c->g.on) continue;
+            c->g.it += step;
+            if (c->g.it % c->g.B) continue;
+            if ((rc = gz_block_end(c))) return bfail(b, rc, std::string("member p=") + std::to_string(c->p) + ": " + c->err);
+            BHIPCHK(b, hipEventRecord(c->ev_gz, c->stream), "hipEventRecord");
+            BHIPCHK(b, hipStreamWaitEvent(b->stream, c->ev_gz, 0), "hipStreamWaitEvent");
+        }
     }
     return LNT_OK;
 }
@@ -999,8 +1048,10 @@ void lnt_batch_destroy(lnt_batch *b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (lnt_ctx *c : b->members) c->batch = nullptr;
     batch_release(b);
     if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->done) (void)hipEventDestroy(b->done);
     if (b->ev_mark) (void)hipEventDestroy(b->ev_mark);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;

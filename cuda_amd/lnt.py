@@ -33,7 +33,9 @@ EXPORTS = (
     "lnt_length", "lnt_describe", "lnt_profile", "lnt_mul", "lnt_copy", "lnt_inject_error",
     "lnt_gerbicz_begin", "lnt_gerbicz_iterate", "lnt_gerbicz_stats", "lnt_batch_create",
     "lnt_batch_add", "lnt_batch_remove", "lnt_batch_size", "lnt_batch_iterate", "lnt_batch_sync",
-    "lnt_batch_destroy", "lnt_batch_last_error",
+    "lnt_batch_destroy", "lnt_batch_last_error", "lnt_server_create", "lnt_server_submit",
+    "lnt_server_poll", "lnt_server_stats", "lnt_server_now", "lnt_server_destroy",
+    "lnt_server_last_error",
 )
 
 
@@ -46,6 +48,21 @@ class LntError(RuntimeError):
 
 
 _lib = None
+
+
+class ServerConfig(ctypes.Structure):
+    """lnt_server_config"""
+    _fields_ = [("device", ctypes.c_int), ("hbm_budget", ctypes.c_uint64), ("max_tests", ctypes.c_uint32),
+                ("step_ms", ctypes.c_float), ("gerbicz_block", ctypes.c_uint32),
+                ("gerbicz_check", ctypes.c_uint32)]
+
+
+class Result(ctypes.Structure):
+    """lnt_result"""
+    _fields_ = [("tag", ctypes.c_uint64), ("p", ctypes.c_uint32), ("prp", ctypes.c_int32),
+                ("iterations", ctypes.c_uint64), ("res64", ctypes.c_uint64), ("is_zero", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double),
+                ("done_s", ctypes.c_double), ("checks", ctypes.c_uint64), ("rollbacks", ctypes.c_uint64)]
 
 
 def load(path: str = LIB_PATH) -> ctypes.CDLL:
@@ -93,6 +110,16 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lnt_batch_destroy.restype = None
     lib.lnt_batch_last_error.argtypes = [vp]
     lib.lnt_batch_last_error.restype = ctypes.c_char_p
+    lib.lnt_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+    lib.lnt_server_submit.argtypes = [vp, u32, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64]
+    lib.lnt_server_poll.argtypes = [vp, ctypes.POINTER(Result), ctypes.c_int, ctypes.c_int]
+    lib.lnt_server_stats.argtypes = [vp, u64p, ctypes.c_int]
+    lib.lnt_server_now.argtypes = [vp]
+    lib.lnt_server_now.restype = ctypes.c_double
+    lib.lnt_server_destroy.argtypes = [vp]
+    lib.lnt_server_destroy.restype = None
+    lib.lnt_server_last_error.argtypes = [vp]
+    lib.lnt_server_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
 
@@ -273,9 +300,13 @@ class Batch:
         self._check(self._lib.lnt_batch_remove(self._h, t._h))
         self.members.remove(t)
 
+    def __len__(self) -> int:
+        return int(self._lib.lnt_batch_size(self._h))
+
     def iterate(self, k: int) -> None:
-        """Every member advances k iterations (asynchronous; call sync() before using a
-        member on its own)."""
+        """Every member advances k iterations (asynchronous; a member's own later work is
+        ordered after it)."""
+        self.members = [t for t in self.members if getattr(t, "_h", None)]  # closed members left
         self._check(self._lib.lnt_batch_iterate(self._h, int(k)))
         for t in self.members:
             t.iteration += int(k)
@@ -288,6 +319,53 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "_h", None):
             self._lib.lnt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving loop (lnt_server_*): submit tests at any time; a scheduler thread in the
+    library admits them by device footprint and runs every admitted test in one batch."""
+
+    def __init__(self, hbm_budget: int = 200 << 30, max_tests: int = 0, step_ms: float = 5.0,
+                 gerbicz_block: int = 1000, gerbicz_check: int = 10, device: int = 0):
+        self._lib = load()
+        cfg = ServerConfig(device, hbm_budget, max_tests, step_ms, gerbicz_block, gerbicz_check)
+        h = ctypes.c_void_p()
+        rc = self._lib.lnt_server_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != LNT_OK:
+            raise LntError(rc, self._lib.lnt_last_error(None).decode())
+        self._h = h
+
+    def submit(self, p: int, prp: bool, iterations: int, tag: int) -> None:
+        rc = self._lib.lnt_server_submit(self._h, p, int(prp), iterations, tag)
+        if rc != LNT_OK:
+            raise LntError(rc, self._lib.lnt_server_last_error(self._h).decode())
+
+    def poll(self, wait_ms: int = 0, cap: int = 256) -> List[dict]:
+        buf = (Result * cap)()
+        k = self._lib.lnt_server_poll(self._h, buf, cap, wait_ms)
+        if k < 0:
+            raise LntError(-k, "lnt_server_poll: bad argument")
+        return [{f: getattr(buf[i], f) for f, _ in Result._fields_} for i in range(k)]
+
+    def stats(self) -> dict:
+        out = (ctypes.c_uint64 * 7)()
+        self._lib.lnt_server_stats(self._h, out, 7)
+        keys = ("queued", "running", "finished", "bytes_admitted", "steps", "iterations", "admitted")
+        return {k: int(v) for k, v in zip(keys, out)}
+
+    def now(self) -> float:
+        return float(self._lib.lnt_server_now(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.lnt_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

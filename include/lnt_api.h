@@ -115,8 +115,9 @@ int lnt_gerbicz_stats(lnt_ctx *ctx, uint64_t *out, int cap);
 
 /* ---- batches of independent tests (new: the reference runs one test per process) ----
  * Members keep their own residues; members of the same shape run as one batched launch
- * per kernel and one iteration of all members is a single hipGraph launch.  Between
- * lnt_batch_iterate and lnt_batch_sync a member must not be used on its own. */
+ * per kernel and one iteration of all members is a single hipGraph launch.  Work issued
+ * on a member after lnt_batch_iterate runs after the batch's work.  A context belongs to
+ * at most one batch; lnt_destroy takes it out of its batch. */
 typedef struct lnt_batch lnt_batch;
 int lnt_batch_create(int device, lnt_batch **out);
 int lnt_batch_add(lnt_batch *batch, lnt_ctx *ctx);
@@ -128,6 +129,45 @@ int lnt_batch_iterate(lnt_batch *batch, uint32_t k);
 int lnt_batch_sync(lnt_batch *batch, float *elapsed_ms);
 void lnt_batch_destroy(lnt_batch *batch);
 const char *lnt_batch_last_error(const lnt_batch *batch);
+
+/* ---- serving loop (new: the reference tests one exponent per process) ----
+ * A scheduler thread admits submitted tests while their device footprint fits the
+ * budget, advances all admitted tests as one batch, lets arrivals join and finished tests
+ * leave between scheduling steps (continuous batching), and queues each result with its
+ * timings.  Residues and Gerbicz checkpoints of admitted tests stay on the device. */
+typedef struct lnt_server lnt_server;
+typedef struct lnt_server_config {
+    int device;
+    uint64_t hbm_budget;     /* device bytes the admitted tests may hold */
+    uint32_t max_tests;      /* admitted tests at most (0: no limit) */
+    float step_ms;           /* target GPU time of one scheduling step (0: 5 ms) */
+    uint32_t gerbicz_block;  /* PRP tests: Gerbicz block (0: unchecked) */
+    uint32_t gerbicz_check;  /* blocks per check (0: 10) */
+} lnt_server_config;
+typedef struct lnt_result {
+    uint64_t tag;            /* the caller's id given to lnt_server_submit */
+    uint32_t p;
+    int32_t prp;
+    uint64_t iterations;
+    uint64_t res64;
+    int32_t is_zero;         /* LL: the whole residue is 0 (M_p prime when iterations = p - 2) */
+    int32_t status;          /* lnt_status of this test */
+    double submit_s, admit_s, done_s;  /* seconds since lnt_server_create */
+    uint64_t checks, rollbacks;        /* Gerbicz checks passed / rollbacks (PRP) */
+} lnt_result;
+int lnt_server_create(const lnt_server_config *cfg, lnt_server **out);
+/* Queue a test: `iterations` squarings from the start residue (LL: 4, PRP: 3). */
+int lnt_server_submit(lnt_server *server, uint32_t p, int prp, uint64_t iterations, uint64_t tag);
+/* Move up to cap finished results to out, waiting up to wait_ms for the first; returns
+ * the count (negative lnt_status on a bad argument). */
+int lnt_server_poll(lnt_server *server, lnt_result *out, int cap, int wait_ms);
+/* out[0] queued, [1] admitted and running, [2] finished, [3] device bytes admitted,
+ * [4] scheduling steps, [5] iterations per member summed over steps, [6] admitted. cap >= 7. */
+int lnt_server_stats(lnt_server *server, uint64_t *out, int cap);
+double lnt_server_now(lnt_server *server);
+/* Stop the scheduler; tests still running end with status LNT_INVALID_ARGUMENT. */
+void lnt_server_destroy(lnt_server *server);
+const char *lnt_server_last_error(const lnt_server *server);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,10 @@ NTT: LL s_0 = 4, s_k = s_{k-1}^2 - 2 mod M_p; PRP s_0 = 3, s_k = s_{k-1}^2 mod M
   python tests/golden/make_fixtures.py --import-log 70000027 LL 20000 out.txt
         # merge the "k res64" lines of `oracle/_build/gmp_ll 70000027 LL 20000 1000 > out.txt`
         # (a long run made in the background) into the fixtures
+  python tests/golden/make_fixtures.py --batched 4 > batched.txt
+        # res64 after 10,000 LL iterations of the 64 batched-workload exponents (seed 0),
+        # 4 GMP processes (about an hour); prints "p res64" lines as they finish, then merges
+  python tests/golden/make_fixtures.py --import-batched batched.txt
 
 Output: tests/golden/res64.json
 """
@@ -72,12 +76,50 @@ def gmp_gerbicz(p: int, iters: int, block: int):
     return res, dres
 
 
+BATCHED_ITERS = 10000
+
+
+def _batched_one(p: int):
+    return p, gmp(p, "LL", BATCHED_ITERS, BATCHED_ITERS)[str(BATCHED_ITERS)]
+
+
+def batched(workers: int, path) -> int:
+    """GMP res64 of the batched workload (64 LL tests, seed 0) after 10,000 iterations."""
+    if path:
+        rows = [line.split() for line in open(path) if len(line.split()) == 2]
+        found = {int(a): b for a, b in rows}
+    else:
+        sys.path.insert(0, ROOT)
+        from multiprocessing import Pool
+        from cuda_amd.workloads import batched_exponents
+        ps = sorted(set(batched_exponents(0)))
+        found = {}
+        with Pool(workers) as pool:
+            for p, r in pool.imap_unordered(_batched_one, ps):
+                print(p, r, flush=True)
+                found[p] = r
+    doc = json.load(open(OUT))
+    have = {(c["p"], c["mode"], c["iters"]) for c in doc["cases"]}
+    for p, r in sorted(found.items()):
+        if (p, "LL", BATCHED_ITERS) not in have:
+            doc["cases"].append({"p": p, "mode": "LL", "iters": BATCHED_ITERS, "batched": True,
+                                 "res64": {str(BATCHED_ITERS): r}})
+    with open(OUT, "w") as f:
+        json.dump(doc, f, indent=1, sort_keys=True)
+    print(f"merged {len(found)} batched-workload fixtures", file=sys.stderr)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--long", action="store_true", help="also run the multi-hour p=70M/140M fixtures")
     ap.add_argument("--import-log", nargs=4, metavar=("P", "MODE", "ITERS", "FILE"))
     ap.add_argument("--block", type=int, default=0, help="with --import-log: the log has Gerbicz columns")
+    ap.add_argument("--batched", type=int, default=0, metavar="WORKERS")
+    ap.add_argument("--import-batched", metavar="FILE")
     args = ap.parse_args()
+    if args.batched or args.import_batched:
+        return batched(args.batched, args.import_batched)
     if args.import_log:
         p, mode, iters, path = args.import_log
         doc = json.load(open(OUT))
@@ -125,6 +167,7 @@ def main() -> int:
         cases.append({"p": 140000041, "mode": "PRP", "iters": 10000, "block": 1000, "res64": res, "dres64": dres})
     elif prev:
         cases.append(prev[0])
+    cases += [c for c in old["cases"] if c.get("batched")]  # made by --batched (an hour)
     doc = {
         "generator": "tests/golden/make_fixtures.py (oracle/gmp_ll.c, GMP mpz)",
         "definition": "LL: s0=4, s=s^2-2 mod 2^p-1; PRP: s0=3, s=s^2 mod 2^p-1; res64 = low 64 bits "
