@@ -21,6 +21,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms/iter @2^22-word NTT (+% HBM roofline)"
 HBM_PEAK_GBS = 8000.0
+# integer VALU: 256 CUs x 64 lanes x 2.4 GHz (one 32-bit lane-op per lane per clock)
+VALU_PEAK = 256 * 64 * 2.4e9
 
 
 def is_prime(x: int) -> bool:
@@ -145,6 +147,9 @@ def main() -> int:
     ap.add_argument("--profile-iters", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--exponent", type=int, default=0,
+                    help="measure this exponent instead (e.g. 140000041 with --prp: the 2^23 PRP test)")
+    ap.add_argument("--prp", action="store_true", help="PRP (start 3, no -2) instead of LL")
     ap.add_argument("--batch-iters", type=int, default=10000,
                     help="iterations per test of the batched workload (0: skip that leg)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -172,8 +177,8 @@ def main() -> int:
             f"(reads {8 * words} B, writes {8 * words} B per launch)")
         del a, b
 
-    p = next_prime(70_000_000 + args.seed % 100_000)
-    t = LucasPRP(p)
+    p = args.exponent or next_prime(70_000_000 + args.seed % 100_000)
+    t = LucasPRP(p, prp=args.prp)
     n = t.n
     log(f"bench: p={p} n={n} plan={t.describe()}")
 
@@ -193,7 +198,7 @@ def main() -> int:
     try:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from helpers import golden
-        g = golden(p, "LL") or {}
+        g = golden(p, "PRP" if args.prp else "LL") or {}
         if done in g:
             check = t.res64() == g[done]
     except Exception as e:  # fixtures are optional for the measurement
@@ -204,13 +209,21 @@ def main() -> int:
     alg_bytes = 16 * n  # one read and one write of the n-word residue per launch
     achieved = alg_bytes / (kms * 1e-3) / 1e9
     iter_ms = sum(v for _, v in prof)
-    traffic = None
+    # HBM bytes and VALU lane-ops per launch of each kernel, from the rocprofv3 counter runs
+    # summarised in profiles/ (FETCH_SIZE / WRITE_SIZE scaled by the copy calibration)
+    counters = {}
     tfile = os.path.join(ROOT, "profiles", "traffic_2p22.json")
-    if os.path.exists(tfile):
+    if n == 1 << 22 and os.path.exists(tfile):
         try:
-            traffic = json.load(open(tfile)).get(name.split("<")[0])
+            counters = json.load(open(tfile)).get("kernels", {}).get(name.split("<")[0], {})
         except Exception:
-            traffic = None
+            counters = {}
+    traffic = counters.get("traffic_bytes")
+    valu = None
+    if counters.get("valu_lane_ops"):
+        va = counters["valu_lane_ops"] / (kms * 1e-3)
+        valu = {"achieved": round(va / 1e12, 2), "peak": VALU_PEAK / 1e12, "unit": "T lane-ops/s",
+                "frac": round(va / VALU_PEAK, 4), "lane_ops_per_launch": counters["valu_lane_ops"]}
     line = {
         "metric": METRIC,
         "value": round(ms_per_step, 5),
@@ -224,8 +237,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": f"LL test of M{p}, n=2^22 u64 words (GF(2^64-2^32+1) IBDWT), "
-                               f"iterations {args.warmup + 1}..{done}",
+        "config": {"workload": f"{'PRP' if args.prp else 'LL'} test of M{p}, n=2^{n.bit_length() - 1} u64 "
+                               f"words (GF(2^64-2^32+1) IBDWT), iterations {args.warmup + 1}..{done}",
                    "p": p, "n": n, "plan": t.describe()},
         "gpu_ms_per_step": round(gpu_ms_per_step, 5),
         "res64": f"{t.res64():016X}",
@@ -233,7 +246,7 @@ def main() -> int:
         "kernels_ms": {k: round(v, 5) for k, v in prof},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic,
+                     "traffic": traffic, "valu": valu,
                      "iteration": {"bytes": 32 * n, "ms": round(iter_ms, 5),
                                    "achieved": round(32 * n / (iter_ms * 1e-3) / 1e9, 1),
                                    "frac": round(32 * n / (iter_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},

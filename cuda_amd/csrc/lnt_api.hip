@@ -44,6 +44,12 @@ __global__ void lnt_k_tw4(u64 om, u32 C, u32 B1, u32 B2, u32 n, u64 *tw) {
     tw[i] = lnt_tw4(om, C, B1, B2, i);
 }
 
+__global__ void lnt_k_tw4_3(u64 om, u32 C, u32 BA, u32 BB, u32 BC, u32 n, u64 *tw) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    tw[i] = lnt_tw4_3(om, C, BA, BB, BC, i);
+}
+
 // ------------------------------------------------------------------ context
 
 struct lnt_ctx {
@@ -62,6 +68,7 @@ struct lnt_ctx {
     std::vector<uint8_t> wid;
     u64 *x = nullptr, *y = nullptr, *w = nullptr, *uw = nullptr;
     u64 *tw4 = nullptr, *itw4 = nullptr, *twr = nullptr, *itwr = nullptr, *twc = nullptr, *itwc = nullptr;
+    u64 *twc2 = nullptr, *itwc2 = nullptr;
     u64 *twr2 = nullptr, *itwr2 = nullptr;
     u64 *f = nullptr;  // transform of the second operand of a product (allocated on first use)
     LntParams P{};
@@ -104,7 +111,7 @@ static int fail(lnt_ctx *c, int code, const std::string &msg) {
 static void free_buffers(lnt_ctx *c) {
     u64 **bufs[] = {&c->x,      &c->y,      &c->w,      &c->uw,      &c->tw4,     &c->itw4,   &c->twr,
                     &c->itwr,   &c->twc,    &c->itwc,   &c->f,       &c->g.U0.x,  &c->g.D.x,  &c->g.Dprev.x,
-                    &c->g.E.x,  &c->g.Uv.x, &c->g.Dv.x, &c->twr2, &c->itwr2};
+                    &c->g.E.x,  &c->g.Uv.x, &c->g.Dv.x, &c->twr2, &c->itwr2, &c->twc2, &c->itwc2};
     for (u64 **b : bufs)
         if (*b) {
             (void)hipFree(*b);
@@ -146,15 +153,35 @@ static int build_tables(lnt_ctx *c) {
                        c->itwr);
     if (!pl.small) {
         u64 omc = gl_pow(554, (GL_P - 1) / pl.R);
-        hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, c->stream, omc, pl.col.B1,
-                           pl.col.B2, pl.R, c->twc);
-        hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, c->stream, gl_inv(omc), pl.col.B1,
-                           pl.col.B2, pl.R, c->itwc);
         u64 omn = gl_pow(554, (GL_P - 1) / n);
-        hipLaunchKernelGGL(lnt_k_tw4, dim3((n + 255) / 256), dim3(256), 0, c->stream, omn, pl.C, pl.col.B1,
-                           pl.col.B2, n, c->tw4);
-        hipLaunchKernelGGL(lnt_k_tw4, dim3((n + 255) / 256), dim3(256), 0, c->stream, gl_inv(omn), pl.C, pl.col.B1,
-                           pl.col.B2, n, c->itw4);
+        if (pl.col3) {
+            // phase 1 (BA x A line) and phase 2 (BB x BC line) twiddles, 3-phase slot order
+            const u32 A = pl.R / pl.col3;
+            HIPCHK(c, hipMalloc(&c->twc2, (size_t)A * 8), "hipMalloc (column twiddles)");
+            HIPCHK(c, hipMalloc(&c->itwc2, (size_t)A * 8), "hipMalloc (column twiddles)");
+            hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, c->stream, omc, pl.col3, A, pl.R,
+                               c->twc);
+            hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, c->stream, gl_inv(omc), pl.col3,
+                               A, pl.R, c->itwc);
+            u64 oma = gl_pow(554, (GL_P - 1) / A);
+            hipLaunchKernelGGL(lnt_k_line_tw, dim3((A + 255) / 256), dim3(256), 0, c->stream, oma, pl.cBB, pl.cBC, A,
+                               c->twc2);
+            hipLaunchKernelGGL(lnt_k_line_tw, dim3((A + 255) / 256), dim3(256), 0, c->stream, gl_inv(oma), pl.cBB,
+                               pl.cBC, A, c->itwc2);
+            hipLaunchKernelGGL(lnt_k_tw4_3, dim3((n + 255) / 256), dim3(256), 0, c->stream, omn, pl.C, pl.col3, pl.cBB,
+                               pl.cBC, n, c->tw4);
+            hipLaunchKernelGGL(lnt_k_tw4_3, dim3((n + 255) / 256), dim3(256), 0, c->stream, gl_inv(omn), pl.C, pl.col3,
+                               pl.cBB, pl.cBC, n, c->itw4);
+        } else {
+            hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, c->stream, omc, pl.col.B1,
+                               pl.col.B2, pl.R, c->twc);
+            hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, c->stream, gl_inv(omc),
+                               pl.col.B1, pl.col.B2, pl.R, c->itwc);
+            hipLaunchKernelGGL(lnt_k_tw4, dim3((n + 255) / 256), dim3(256), 0, c->stream, omn, pl.C, pl.col.B1,
+                               pl.col.B2, n, c->tw4);
+            hipLaunchKernelGGL(lnt_k_tw4, dim3((n + 255) / 256), dim3(256), 0, c->stream, gl_inv(omn), pl.C,
+                               pl.col.B1, pl.col.B2, n, c->itw4);
+        }
     }
     if (pl.row3) {
         const u32 L2 = pl.row3 * pl.row3;
@@ -180,6 +207,8 @@ static int build_tables(lnt_ctx *c) {
     P.itwr = view<const u64>(c->itwr, Lr);
     P.twc = view<const u64>(c->twc, pl.small ? 0 : pl.R);
     P.itwc = view<const u64>(c->itwc, pl.small ? 0 : pl.R);
+    P.twc2 = view<const u64>(c->twc2, pl.col3 ? pl.R / pl.col3 : 0);
+    P.itwc2 = view<const u64>(c->itwc2, pl.col3 ? pl.R / pl.col3 : 0);
     P.p = c->p;
     P.n = n;
     P.log_n = pl.log_n;
@@ -310,6 +339,7 @@ static bool env_plan(PlanOpts &o) {
             if (k == "lc") o.lc = (uint32_t)v;
             if (k == "K") o.K = (uint32_t)v;
             if (k == "row3") o.row3 = v;
+            if (k == "col3") o.col3 = v;
         }
         if (q == std::string::npos) break;
         pos = q + 1;
@@ -481,8 +511,12 @@ static std::string describe(const Plan &pl) {
         char row[64];
         if (pl.row3) std::snprintf(row, sizeof row, "lnt_k_row3<%u,%u>", pl.row3, pl.G);
         else std::snprintf(row, sizeof row, "lnt_k_row<%u,%u>", pl.row.B1, pl.row.B2);
-        std::snprintf(b, sizeof b, "LNT_2PASS_%ux%u;lnt_k_col_fwd<%u,%u,%u>;%s;lnt_k_col_inv<%u,%u,%u>", pl.R, pl.C,
-                      pl.col.B1, pl.col.B2, pl.K, row, pl.col.B1, pl.col.B2, pl.K);
+        if (pl.col3)
+            std::snprintf(b, sizeof b, "LNT_2PASS_%ux%u;lnt_k_col3_fwd<%u,%u,%u,%u>;%s;lnt_k_col3_inv<%u,%u,%u,%u>", pl.R,
+                          pl.C, pl.col3, pl.cBB, pl.cBC, pl.K, row, pl.col3, pl.cBB, pl.cBC, pl.K);
+        else
+            std::snprintf(b, sizeof b, "LNT_2PASS_%ux%u;lnt_k_col_fwd<%u,%u,%u>;%s;lnt_k_col_inv<%u,%u,%u>", pl.R,
+                          pl.C, pl.col.B1, pl.col.B2, pl.K, row, pl.col.B1, pl.col.B2, pl.K);
     }
     return b;
 }
@@ -500,33 +534,33 @@ int lnt_profile(lnt_ctx *c, uint32_t iters, char *names, size_t names_len, float
     const Plan &pl = c->pl;
     const bool ll = !c->prp;
     const int nk = pl.small ? 1 : 3;
-    std::vector<hipEvent_t> ev((size_t)nk * iters + 1);
+    // Each kernel runs `iters` times back to back between two events (the residue is
+    // saved first and restored after: repeated launches of one kernel are timing only).
+    u64 *save = nullptr;
+    HIPCHK(c, hipMalloc(&save, (size_t)pl.n * 8), "hipMalloc (profile)");
+    HIPCHK(c, hipMemcpyAsync(save, c->x, (size_t)pl.n * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpy");
+    std::vector<hipEvent_t> ev((size_t)nk * 2);
     for (auto &e : ev) HIPCHK(c, hipEventCreate(&e), "hipEventCreate");
-    HIPCHK(c, hipEventRecord(ev[0], c->stream), "hipEventRecord");
-    size_t ei = 1;
-    for (uint32_t i = 0; i < iters; i++) {
-        if (pl.small) {
-            lnt_launch_small(pl, c->P, 1, ll && c->pending, ll, c->stream);
-            c->pending = true;
-            HIPCHK(c, hipEventRecord(ev[ei++], c->stream), "hipEventRecord");
-        } else {
-            lnt_launch_col_fwd(pl, c->P, ll && c->pending, c->stream);
-            c->pending = true;
-            HIPCHK(c, hipEventRecord(ev[ei++], c->stream), "hipEventRecord");
-            lnt_launch_row(pl, c->P, c->stream);
-            HIPCHK(c, hipEventRecord(ev[ei++], c->stream), "hipEventRecord");
-            lnt_launch_col_inv(pl, c->P, c->stream);
-            HIPCHK(c, hipEventRecord(ev[ei++], c->stream), "hipEventRecord");
+    const bool sub2 = ll && c->pending;
+    for (int k = 0; k < nk; k++) {
+        HIPCHK(c, hipEventRecord(ev[2 * k], c->stream), "hipEventRecord");
+        for (uint32_t i = 0; i < iters; i++) {
+            if (pl.small) lnt_launch_small(pl, c->P, 1, sub2, ll, c->stream);
+            else if (k == 0) lnt_launch_col_fwd(pl, c->P, sub2, c->stream);
+            else if (k == 1) lnt_launch_row(pl, c->P, c->stream);
+            else lnt_launch_col_inv(pl, c->P, c->stream);
         }
+        HIPCHK(c, hipEventRecord(ev[2 * k + 1], c->stream), "hipEventRecord");
     }
-    c->pristine = false;
     HIPCHK(c, hipGetLastError(), "kernel launch");
-    HIPCHK(c, hipEventSynchronize(ev.back()), "hipEventSynchronize");
+    HIPCHK(c, hipMemcpyAsync(c->x, save, (size_t)pl.n * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpy");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    (void)hipFree(save);
     std::vector<double> tot(nk, 0.0);
-    for (size_t i = 1; i < ev.size(); i++) {
+    for (int k = 0; k < nk; k++) {
         float t = 0;
-        HIPCHK(c, hipEventElapsedTime(&t, ev[i - 1], ev[i]), "hipEventElapsedTime");
-        tot[(i - 1) % nk] += t;
+        HIPCHK(c, hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]), "hipEventElapsedTime");
+        tot[k] = t;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     std::string d = describe(pl);
@@ -569,14 +603,16 @@ int lnt_tune(lnt_ctx *c, int loops) {
     std::vector<PlanOpts> opts;
     for (uint32_t lc = 7; lc <= 12; lc++)
         for (uint32_t K : {16u, 8u, 4u})
-            for (int r3 : {0, 1}) {
-                PlanOpts o;
-                o.lc = lc;
-                o.K = K;
-                o.row3 = r3;
-                Plan tmp;
-                if (make_plan(c->p, n, tmp, o).empty()) opts.push_back(o);
-            }
+            for (int r3 : {0, 1})
+                for (int c3 : {0, 1}) {
+                    PlanOpts o;
+                    o.lc = lc;
+                    o.K = K;
+                    o.row3 = r3;
+                    o.col3 = c3;
+                    Plan tmp;
+                    if (make_plan(c->p, n, tmp, o).empty()) opts.push_back(o);
+                }
     std::vector<Cand> cands;
     const uint32_t warm = std::max(2, loops / 10);
     for (const PlanOpts &o : opts) {
@@ -616,7 +652,7 @@ int lnt_tune(lnt_ctx *c, int loops) {
     }
     Plan bp;
     make_plan(c->p, n, bp, best->o);
-    if (bp.C == c->pl.C && bp.K == c->pl.K && bp.row3 == c->pl.row3) return LNT_OK;
+    if (bp.C == c->pl.C && bp.K == c->pl.K && bp.row3 == c->pl.row3 && bp.col3 == c->pl.col3) return LNT_OK;
     // rebuild this context on the winning plan, keeping the residue
     lnt_ctx *nc = nullptr;
     int rc = make_context(c->device, c->p, n, c->prp, &best->o, &nc);
@@ -636,6 +672,8 @@ int lnt_tune(lnt_ctx *c, int loops) {
     std::swap(c->itwr2, nc->itwr2);
     std::swap(c->twc, nc->twc);
     std::swap(c->itwc, nc->itwc);
+    std::swap(c->twc2, nc->twc2);
+    std::swap(c->itwc2, nc->itwc2);
     std::swap(c->P, nc->P);
     lnt_destroy(nc);
     return LNT_OK;
@@ -644,8 +682,9 @@ int lnt_tune(lnt_ctx *c, int loops) {
 // ------------------------------------------------------------------ products, copies, faults
 
 static bool same_shape(const lnt_ctx *a, const lnt_ctx *b) {
+    // the transform-domain slot order depends on the split and on the row/column variants
     return a->device == b->device && a->p == b->p && a->prp == b->prp && a->pl.n == b->pl.n && a->pl.C == b->pl.C &&
-           a->pl.small == b->pl.small;
+           a->pl.small == b->pl.small && a->pl.row3 == b->pl.row3 && a->pl.col3 == b->pl.col3;
 }
 
 int lnt_mul(lnt_ctx *dst, lnt_ctx *src) {
@@ -844,7 +883,8 @@ static int bfail(lnt_batch *b, int code, const std::string &m) {
     } while (0)
 
 static bool same_group(const Plan &a, const Plan &b) {
-    return a.small == b.small && a.n == b.n && a.C == b.C && a.K == b.K && a.row3 == b.row3 && a.M == b.M;
+    return a.small == b.small && a.n == b.n && a.C == b.C && a.K == b.K && a.row3 == b.row3 && a.col3 == b.col3 &&
+           a.M == b.M;
 }
 
 static void batch_release(lnt_batch *b) {

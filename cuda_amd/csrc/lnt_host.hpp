@@ -57,6 +57,8 @@ struct Plan {
     uint32_t K = 0;         // columns per column tile
     uint32_t G = 0;         // rows per row workgroup
     uint32_t row3 = 0;      // 3-phase rows: C = row3^3 (0: 2-phase rows)
+    uint32_t col3 = 0;      // 3-phase columns: first radix BA (0: 2-phase columns)
+    uint32_t cBB = 0, cBC = 0;  // 3-phase columns: second and third radix
     uint32_t M = 0;         // carry halo (compile-time), >= levels
     uint32_t levels = 0;    // carry levels
 };
@@ -102,7 +104,16 @@ struct PlanOpts {
     uint32_t lc = 0;    // log2 of the row length
     uint32_t K = 0;     // column tile width
     int row3 = -1;      // 1: 3-phase rows when C is a cube, 0: 2-phase, -1: default
+    int col3 = -1;      // 1: 3-phase columns (R = 512 .. 4096), 0: 2-phase, -1: default
 };
+
+// Column tile widths of the 3-phase column kernels.
+inline bool col3_tile_ok(uint32_t R, uint32_t K) {
+    static const uint32_t shapes[][2] = {{512, 16}, {512, 8}, {1024, 16}, {1024, 8}, {2048, 8}, {2048, 4}, {4096, 4}};
+    for (auto &s : shapes)
+        if (s[0] == R && s[1] == K) return true;
+    return false;
+}
 
 // Column tile widths that fit the column kernels' LDS (R * K words of 8 bytes <= 128 KiB).
 inline bool col_tile_ok(uint32_t R, uint32_t K) {
@@ -144,8 +155,18 @@ inline std::string make_plan(uint32_t p, uint64_t n, Plan &pl, PlanOpts opts = P
     pl.row = line_for(lc);
     pl.col = line_for(lr);
     pl.K = pl.R >= 4096 ? 4 : pl.R >= 2048 ? 8 : 16;
+    // 3-phase columns by default for R >= 2048 (measured faster there; equal at R = 1024)
+    const int col3 = opts.col3 >= 0 ? opts.col3 : (pl.R >= 2048 ? 1 : 0);
+    if (col3 == 1) {
+        if (pl.R < 512 || pl.R > 4096) return "3-phase columns need R = 512 .. 4096";
+        pl.col3 = pl.R >= 1024 ? 16 : 8;
+        const uint32_t A = pl.R / pl.col3;
+        pl.cBB = 1u << ((ilog2(A) + 1) / 2);
+        pl.cBC = A / pl.cBB;
+        pl.K = pl.R >= 4096 ? 4 : 8;  // R = 2048: K = 8 (1024-thread blocks) measured faster than K = 4
+    }
     if (opts.K) {
-        if (!col_tile_ok(pl.R, opts.K)) return "unsupported column tile";
+        if (!(pl.col3 ? col3_tile_ok(pl.R, opts.K) : col_tile_ok(pl.R, opts.K))) return "unsupported column tile";
         pl.K = opts.K;
     }
     pl.G = 64 / pl.row.B2;

@@ -21,7 +21,28 @@ static bool go_col_inv(const lnt::Plan &pl, const LntParams &P, hipStream_t s) {
 #define COL_SHAPES(X) \
     X(8, 8, 16) X(16, 8, 16) X(16, 16, 16) X(32, 16, 16) X(32, 32, 16) X(32, 32, 8) X(64, 32, 8) X(64, 32, 4) X(64, 64, 4)
 
+// 3-phase column shapes (BA, BB, BC, K)
+#define COL3_SHAPES(X) \
+    X(8, 8, 8, 16) X(8, 8, 8, 8) X(16, 8, 8, 16) X(16, 8, 8, 8) X(16, 16, 8, 8) X(16, 16, 8, 4) X(16, 16, 16, 4)
+
+template <int BA, int BB, int BC, int K>
+static bool go_col3_fwd(const lnt::Plan &pl, const LntParams &P, bool sub2, hipStream_t s) {
+    dim3 grid(pl.C / K), block(K * BB * BC);
+    if (pl.M == 4)
+        hipLaunchKernelGGL((lnt_k_col3_fwd<BA, BB, BC, K, 4>), grid, block, 0, s, P, (int)sub2);
+    else
+        hipLaunchKernelGGL((lnt_k_col3_fwd<BA, BB, BC, K, 8>), grid, block, 0, s, P, (int)sub2);
+    return true;
+}
+
 bool lnt_launch_col_fwd(const lnt::Plan &pl, const LntParams &P, bool sub2, hipStream_t s) {
+    if (pl.col3) {
+#define C3F(ba, bb, bc, k) \
+    if (pl.col3 == ba && pl.cBB == bb && pl.cBC == bc && pl.K == k) return go_col3_fwd<ba, bb, bc, k>(pl, P, sub2, s);
+        COL3_SHAPES(C3F)
+#undef C3F
+        return false;
+    }
 #define CF(b1, b2, k) if (pl.col.B1 == b1 && pl.col.B2 == b2 && pl.K == k) return go_col_fwd<b1, b2, k>(pl, P, sub2, s);
     COL_SHAPES(CF)
 #undef CF
@@ -29,6 +50,16 @@ bool lnt_launch_col_fwd(const lnt::Plan &pl, const LntParams &P, bool sub2, hipS
 }
 
 bool lnt_launch_col_inv(const lnt::Plan &pl, const LntParams &P, hipStream_t s) {
+    if (pl.col3) {
+#define C3I(ba, bb, bc, k)                                                                           \
+    if (pl.col3 == ba && pl.cBB == bb && pl.cBC == bc && pl.K == k) {                                \
+        hipLaunchKernelGGL((lnt_k_col3_inv<ba, bb, bc, k>), dim3(pl.C / k), dim3(k * bb * bc), 0, s, P); \
+        return true;                                                                                 \
+    }
+        COL3_SHAPES(C3I)
+#undef C3I
+        return false;
+    }
 #define CI(b1, b2, k) if (pl.col.B1 == b1 && pl.col.B2 == b2 && pl.K == k) return go_col_inv<b1, b2, k>(pl, P, s);
     COL_SHAPES(CI)
 #undef CI
@@ -36,6 +67,20 @@ bool lnt_launch_col_inv(const lnt::Plan &pl, const LntParams &P, hipStream_t s) 
 }
 
 bool lnt_launch_batch_col_fwd(const lnt::Plan &pl, const LntParams *Ps, u32 count, hipStream_t s) {
+    if (pl.col3) {
+#define C3F(ba, bb, bc, k)                                                                                 \
+    if (pl.col3 == ba && pl.cBB == bb && pl.cBC == bc && pl.K == k) {                                      \
+        const u32 wgs = pl.C / k;                                                                          \
+        if (pl.M == 4)                                                                                     \
+            hipLaunchKernelGGL((lnt_kb_col3_fwd<ba, bb, bc, k, 4>), dim3(wgs * count), dim3(k * bb * bc), 0, s, Ps, wgs); \
+        else                                                                                               \
+            hipLaunchKernelGGL((lnt_kb_col3_fwd<ba, bb, bc, k, 8>), dim3(wgs * count), dim3(k * bb * bc), 0, s, Ps, wgs); \
+        return true;                                                                                       \
+    }
+        COL3_SHAPES(C3F)
+#undef C3F
+        return false;
+    }
 #define CF(b1, b2, k)                                                                                     \
     if (pl.col.B1 == b1 && pl.col.B2 == b2 && pl.K == k) {                                                \
         const u32 wgs = pl.C / k;                                                                         \
@@ -51,6 +96,17 @@ bool lnt_launch_batch_col_fwd(const lnt::Plan &pl, const LntParams *Ps, u32 coun
 }
 
 bool lnt_launch_batch_col_inv(const lnt::Plan &pl, const LntParams *Ps, u32 count, hipStream_t s) {
+    if (pl.col3) {
+#define C3I(ba, bb, bc, k)                                                                                \
+    if (pl.col3 == ba && pl.cBB == bb && pl.cBC == bc && pl.K == k) {                                     \
+        const u32 wgs = pl.C / k;                                                                         \
+        hipLaunchKernelGGL((lnt_kb_col3_inv<ba, bb, bc, k>), dim3(wgs * count), dim3(k * bb * bc), 0, s, Ps, wgs); \
+        return true;                                                                                      \
+    }
+        COL3_SHAPES(C3I)
+#undef C3I
+        return false;
+    }
 #define CI(b1, b2, k)                                                                                   \
     if (pl.col.B1 == b1 && pl.col.B2 == b2 && pl.K == k) {                                              \
         const u32 wgs = pl.C / k;                                                                       \

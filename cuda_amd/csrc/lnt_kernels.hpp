@@ -47,6 +47,7 @@ struct LntParams {
     GView<const u64> twr, itwr;  // row-line twiddles (C words)
     GView<const u64> twr2, itwr2;  // second-level row twiddles (3-phase rows: B^2 words)
     GView<const u64> twc, itwc;  // column-line twiddles (R words)
+    GView<const u64> twc2, itwc2;  // second-level column twiddles (3-phase columns: R / BA words)
     GView<const u64> f;      // n words: the other operand of a product, transform domain
     u32 p, n, log_n, C, R;
     u32 levels;              // carry levels m
@@ -243,6 +244,175 @@ LNT_KFN void lnt_col_inv_body(int tid, int bid, LView lds, const LntParams &P) {
     for (int j = 0; j < B1; j++) {
         const u64 idx = c + C * ((u64)t + (u64)B2 * j);
         P.x[idx] = gl_mul(a[j], PF ? uv[PF ? j : 0] : P.uw[idx]);
+    }
+}
+
+// ------------------------------------------------------------------ 3-phase columns
+// Column line R = BA * BB * BC handled by A = BB * BC threads per column (K columns per
+// workgroup, block K * A threads, BA words per thread).  Row r = a + A j (a < A, j < BA):
+//  phase 1: thread a, BA-point DFT over j -> k1, twiddle w_R^(a k1)
+//  phase 2: items (k1, u), u < BC (BA / BB per thread): BB-point DFT over v of the words
+//           a = u + BC v -> k2, twiddle w_A^(u k2)
+//  phase 3: items (k1, k2) (BA / BC per thread): BC-point DFT over u -> k3
+// Frequency k1 + BA (k2 + BB k3) ends in slot t*BA + q*BC + rho (k3 = brev(rho)).
+// LDS word (row, column) at row * KP + column, rows k1 * (A + 1) + a.
+template <int BA, int BB, int BC, int K>
+struct LntCol3 {
+    static constexpr int A = BB * BC, R = BA * A;
+    static constexpr int QB = BA / BB, QC = BA / BC;
+    static constexpr int RS = A + 1;
+    static constexpr int KP = BA * RS * (K + 1) <= 18432 ? K + 1 : K;  // padded when it fits 144 KiB
+    static constexpr int KS = R * (K + 1) <= 18432 ? K + 1 : K;        // carry staging row width
+    static constexpr u32 LDS = (u32)(BA * RS * KP > R * KS ? BA * RS * KP : R * KS);
+    static_assert(BA % BB == 0 && BA % BC == 0 && BB >= BC, "3-phase column radices");
+};
+
+template <int BA, int BB, int BC, int K, int M>
+LNT_KFN void lnt_col3_fwd_body(int tid, int bid, LView lds, const LntParams &P, bool sub2) {
+    using S = LntCol3<BA, BB, BC, K>;
+    constexpr int A = S::A, RR = S::R, RS = S::RS, KP = S::KP, KS = S::KS;
+    constexpr int NT = K * A;
+    constexpr int RT = (RR + NT - 1) / NT;  // carry rows per thread
+    constexpr bool HP = RT * M <= 16;
+    const u64 C = P.C;
+    const u64 n = P.n;
+    const u64 c0 = (u64)bid * K;
+    const int cl = tid % K, t = tid / K;
+    const u64 c = c0 + cl;
+
+    // 0. loads: rows t + A j of column c (K adjacent lanes read one row run), the halo
+    //    digits of this thread's carry rows, the weights (prefetched into registers only
+    //    in blocks of <= 512 threads, whose register budget allows it)
+    constexpr bool PW = NT <= 512;
+    u64 a[BA], wv[PW ? BA : 1], h[HP ? RT : 1][M];
+#pragma unroll
+    for (int j = 0; j < BA; j++) a[j] = P.x[c + C * (u64)(t + A * j)];
+    if (HP) {
+#pragma unroll
+        for (int q = 0; q < RT; q++) {
+            const u64 g0 = C * (u64)(tid + NT * q) + c0;
+#pragma unroll
+            for (int j = 0; j < M; j++)
+                h[HP ? q : 0][j] = (tid + NT * q < RR) ? P.x[(g0 + n - M + j) & (n - 1)] : 0;
+        }
+    }
+    if (PW) {
+#pragma unroll
+        for (int j = 0; j < BA; j++) wv[PW ? j : 0] = P.w[c + C * (u64)(t + A * j)];
+    }
+#pragma unroll
+    for (int j = 0; j < BA; j++) lds[(u32)((t + A * j) * KS + cl)] = a[j];
+    LNT_SYNC();
+
+    // 1. carry + (-2): one thread per row run of K digits (plus the M halo digits)
+#pragma unroll
+    for (int q = 0; q < RT; q++) {
+        const int r = tid + NT * q;
+        if (r >= RR) break;
+        int64_t v[M + K];
+        const u64 g0 = C * (u64)r + c0;
+#pragma unroll
+        for (int j = 0; j < M; j++) v[j] = gl_to_i64(HP ? h[HP ? q : 0][j] : P.x[(g0 + n - M + j) & (n - 1)]);
+#pragma unroll
+        for (int j = 0; j < K; j++) v[M + j] = gl_to_i64(lds[(u32)(r * KS + j)]);
+        lnt_carry_run<M, K>(v, P.p, P.log_n, g0, P.levels, sub2);
+#pragma unroll
+        for (int j = 0; j < K; j++) lds[(u32)(r * KS + j)] = gl_from_i64(v[M + j]);
+    }
+    LNT_SYNC();
+
+    // 2. phase 1: weight, BA-point DFT, twiddle
+#pragma unroll
+    for (int j = 0; j < BA; j++)
+        a[j] = gl_mul(lds[(u32)((t + A * j) * KS + cl)], PW ? wv[PW ? j : 0] : P.w[c + C * (u64)(t + A * j)]);
+    dft_fwd<BA>(a);
+#pragma unroll
+    for (int rho = 1; rho < BA; rho++) a[rho] = gl_mul(a[rho], P.twc[(u64)rho * A + t]);
+    LNT_SYNC();
+#pragma unroll
+    for (int rho = 0; rho < BA; rho++) lds[(u32)((brev_c<BA>(rho) * RS + t) * KP + cl)] = a[rho];
+    LNT_SYNC();
+
+    // 3. phase 2 (each item rewrites exactly the LDS rows it read: no barrier inside)
+#pragma unroll
+    for (int q = 0; q < S::QB; q++) {
+        const int I = q * A + t, k1 = I / BC, u = I % BC;
+        u64 b[BB];
+#pragma unroll
+        for (int v = 0; v < BB; v++) b[v] = lds[(u32)((k1 * RS + u + BC * v) * KP + cl)];
+        dft_fwd<BB>(b);
+#pragma unroll
+        for (int rho = 1; rho < BB; rho++) b[rho] = gl_mul(b[rho], P.twc2[(u64)rho * BC + u]);
+#pragma unroll
+        for (int rho = 0; rho < BB; rho++) lds[(u32)((k1 * RS + BC * brev_c<BB>(rho) + u) * KP + cl)] = b[rho];
+    }
+    LNT_SYNC();
+
+    // 4. phase 3, 4-step twiddle, store slot t*BA + q*BC + rho
+#pragma unroll
+    for (int q = 0; q < S::QC; q++) {
+        const int I = q * A + t, k1 = I / BB, k2 = I % BB;
+#pragma unroll
+        for (int u = 0; u < BC; u++) a[q * BC + u] = lds[(u32)((k1 * RS + BC * k2 + u) * KP + cl)];
+        dft_fwd<BC>(a + q * BC);
+    }
+#pragma unroll
+    for (int i = 0; i < BA; i++) {
+        const u64 idx = c + C * ((u64)t * BA + i);
+        P.y[idx] = gl_mul(a[i], P.tw4[idx]);
+    }
+}
+
+template <int BA, int BB, int BC, int K>
+LNT_KFN void lnt_col3_inv_body(int tid, int bid, LView lds, const LntParams &P) {
+    using S = LntCol3<BA, BB, BC, K>;
+    constexpr int A = S::A, RS = S::RS, KP = S::KP;
+    const u64 C = P.C;
+    const int cl = tid % K, t = tid / K;
+    const u64 c = (u64)bid * K + cl;
+    u64 a[BA], uv[BA];
+#pragma unroll
+    for (int i = 0; i < BA; i++) {
+        const u64 idx = c + C * ((u64)t * BA + i);
+        a[i] = gl_mul(P.y[idx], P.itw4[idx]);
+    }
+#pragma unroll
+    for (int j = 0; j < BA; j++) uv[j] = P.uw[c + C * ((u64)t + (u64)A * j)];
+    // phase 3 inverse
+#pragma unroll
+    for (int q = 0; q < S::QC; q++) {
+        const int I = q * A + t, k1 = I / BB, k2 = I % BB;
+        dft_inv<BC>(a + q * BC);
+#pragma unroll
+        for (int u = 0; u < BC; u++) lds[(u32)((k1 * RS + BC * k2 + u) * KP + cl)] = a[q * BC + u];
+    }
+    LNT_SYNC();
+    // phase 2 inverse
+#pragma unroll
+    for (int q = 0; q < S::QB; q++) {
+        const int I = q * A + t, k1 = I / BC, u = I % BC;
+        u64 b[BB];
+#pragma unroll
+        for (int rho = 0; rho < BB; rho++) {
+            u64 v = lds[(u32)((k1 * RS + BC * brev_c<BB>(rho) + u) * KP + cl)];
+            b[rho] = rho ? gl_mul(v, P.itwc2[(u64)rho * BC + u]) : v;
+        }
+        dft_inv<BB>(b);
+#pragma unroll
+        for (int v = 0; v < BB; v++) lds[(u32)((k1 * RS + u + BC * v) * KP + cl)] = b[v];
+    }
+    LNT_SYNC();
+    // phase 1 inverse, unweight, store
+#pragma unroll
+    for (int rho = 0; rho < BA; rho++) {
+        u64 v = lds[(u32)((brev_c<BA>(rho) * RS + t) * KP + cl)];
+        a[rho] = rho ? gl_mul(v, P.itwc[(u64)rho * A + t]) : v;
+    }
+    dft_inv<BA>(a);
+#pragma unroll
+    for (int j = 0; j < BA; j++) {
+        const u64 idx = c + C * ((u64)t + (u64)A * j);
+        P.x[idx] = gl_mul(a[j], uv[j]);
     }
 }
 
@@ -511,6 +681,18 @@ GL_HD u64 lnt_tw4(u64 omega_n, u32 C, u32 B1, u32 B2, u64 idx) {
     u32 br = 0;
     for (u32 b = 1, v = rp; b < B2; b <<= 1, v >>= 1) br = (br << 1) | (v & 1);
     u64 kcol = (u64)B1 * br + (u64)t * (B1 / B2) + q;
+    return gl_pow(omega_n, (u64)c * kcol);
+}
+
+// 4-step twiddle of a 3-phase column (radices BA, BB, BC): slot s = t*BA + q*BC + rho
+// holds frequency k1 + BA (k2 + BB brev_BC(rho)), (k1, k2) = (I / BB, I % BB), I = q*A + t.
+GL_HD u64 lnt_tw4_3(u64 omega_n, u32 C, u32 BA, u32 BB, u32 BC, u64 idx) {
+    const u32 c = (u32)(idx % C), s = (u32)(idx / C), A = BB * BC;
+    const u32 t = s / BA, q = (s % BA) / BC, rho = s % BC;
+    const u32 I = q * A + t, k1 = I / BB, k2 = I % BB;
+    u32 br = 0;
+    for (u32 b = 1, v = rho; b < BC; b <<= 1, v >>= 1) br = (br << 1) | (v & 1);
+    const u64 kcol = k1 + (u64)BA * (k2 + (u64)BB * br);
     return gl_pow(omega_n, (u64)c * kcol);
 }
 

@@ -37,6 +37,21 @@ __global__ __launch_bounds__(K * B2) void lnt_k_col_inv(LntParams P) {
     lnt_col_inv_body<B1, B2, K>(threadIdx.x, lnt_xcd_bid(blockIdx.x, gridDim.x), LView{smem, W}, P);
 }
 
+template <int BA, int BB, int BC, int K, int M>
+__global__ __launch_bounds__(K * BB * BC) void lnt_k_col3_fwd(LntParams P, int sub2) {
+    constexpr u32 W = LntCol3<BA, BB, BC, K>::LDS;
+    __shared__ u64 smem[W];
+    lnt_col3_fwd_body<BA, BB, BC, K, M>(threadIdx.x, lnt_xcd_bid(blockIdx.x, gridDim.x), LView{smem, W}, P,
+                                        sub2 != 0);
+}
+
+template <int BA, int BB, int BC, int K>
+__global__ __launch_bounds__(K * BB * BC) void lnt_k_col3_inv(LntParams P) {
+    constexpr u32 W = LntCol3<BA, BB, BC, K>::LDS;
+    __shared__ u64 smem[W];
+    lnt_col3_inv_body<BA, BB, BC, K>(threadIdx.x, lnt_xcd_bid(blockIdx.x, gridDim.x), LView{smem, W}, P);
+}
+
 template <int B1, int B2, int G, int MODE>
 __global__ __launch_bounds__(G * B2) void lnt_k_row(LntParams P) {
     constexpr u32 W = G * B1 * (B2 + LNT_PAD);
@@ -78,6 +93,23 @@ __global__ __launch_bounds__(K * B2) void lnt_kb_col_inv(const LntParams *__rest
     __shared__ u64 smem[W];
     const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
     lnt_col_inv_body<B1, B2, K>(threadIdx.x, b % wgs, LView{smem, W}, Ps[b / wgs]);
+}
+
+template <int BA, int BB, int BC, int K, int M>
+__global__ __launch_bounds__(K * BB * BC) void lnt_kb_col3_fwd(const LntParams *__restrict__ Ps, u32 wgs) {
+    constexpr u32 W = LntCol3<BA, BB, BC, K>::LDS;
+    __shared__ u64 smem[W];
+    const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
+    const LntParams &P = Ps[b / wgs];
+    lnt_col3_fwd_body<BA, BB, BC, K, M>(threadIdx.x, b % wgs, LView{smem, W}, P, P.sub2 != 0);
+}
+
+template <int BA, int BB, int BC, int K>
+__global__ __launch_bounds__(K * BB * BC) void lnt_kb_col3_inv(const LntParams *__restrict__ Ps, u32 wgs) {
+    constexpr u32 W = LntCol3<BA, BB, BC, K>::LDS;
+    __shared__ u64 smem[W];
+    const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
+    lnt_col3_inv_body<BA, BB, BC, K>(threadIdx.x, b % wgs, LView{smem, W}, Ps[b / wgs]);
 }
 
 template <int B1, int B2, int G>

@@ -263,7 +263,8 @@ class LucasPRP:
         return buf.value.decode().split(";")
 
     def profile(self, iters: int) -> List[Tuple[str, float]]:
-        """Average GPU ms per launch of each kernel over `iters` iterations."""
+        """Average GPU ms per launch of each kernel of the iteration, each launched `iters`
+        times back to back (the residue is left unchanged)."""
         names = ctypes.create_string_buffer(1024)
         ms = (ctypes.c_float * 8)()
         cnt = self._lib.lnt_profile(self._h, int(iters), names, len(names), ms, 8)

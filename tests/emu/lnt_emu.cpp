@@ -21,7 +21,7 @@ struct Emu {
     Plan pl;
     uint32_t p;
     bool prp, pending;
-    std::vector<uint64_t> x, y, f, w, uw, tw4, itw4, twr, itwr, twr2, itwr2, twc, itwc;
+    std::vector<uint64_t> x, y, f, w, uw, tw4, itw4, twr, itwr, twr2, itwr2, twc, itwc, twc2, itwc2;
     std::vector<uint8_t> wid;
     LntParams P;
 };
@@ -41,7 +41,7 @@ static void line_tables(uint32_t L, Line ln, std::vector<uint64_t> &tw, std::vec
     }
 }
 
-extern "C" void *emu_create2(uint32_t p, uint32_t n, int prp, uint32_t lc, uint32_t K, int row3) {
+extern "C" void *emu_create3(uint32_t p, uint32_t n, int prp, uint32_t lc, uint32_t K, int row3, int col3) {
     Emu *e = new Emu;
     e->p = p;
     e->prp = prp;
@@ -49,6 +49,7 @@ extern "C" void *emu_create2(uint32_t p, uint32_t n, int prp, uint32_t lc, uint3
     o.lc = lc;
     o.K = K;
     o.row3 = row3;
+    o.col3 = col3;
     std::string err = make_plan(p, n, e->pl, o);
     if (!err.empty()) {
         std::fprintf(stderr, "emu_create: %s\n", err.c_str());
@@ -80,7 +81,24 @@ extern "C" void *emu_create2(uint32_t p, uint32_t n, int prp, uint32_t lc, uint3
         e->twr2.assign(1, 1);
         e->itwr2.assign(1, 1);
     }
-    if (!pl.small) {
+    e->twc2.assign(1, 1);
+    e->itwc2.assign(1, 1);
+    if (!pl.small && pl.col3) {
+        Line l1, l2;
+        l1.B1 = pl.col3;
+        l1.B2 = pl.R / pl.col3;
+        l2.B1 = pl.cBB;
+        l2.B2 = pl.cBC;
+        line_tables(pl.R, l1, e->twc, e->itwc);
+        line_tables(pl.R / pl.col3, l2, e->twc2, e->itwc2);
+        e->tw4.resize(n);
+        e->itw4.resize(n);
+        u64 om = gl_pow(554, (GL_P - 1) / n), iom = gl_inv(om);
+        for (uint64_t i = 0; i < n; i++) {
+            e->tw4[i] = lnt_tw4_3(om, pl.C, pl.col3, pl.cBB, pl.cBC, i);
+            e->itw4[i] = lnt_tw4_3(iom, pl.C, pl.col3, pl.cBB, pl.cBC, i);
+        }
+    } else if (!pl.small) {
         line_tables(pl.R, pl.col, e->twc, e->itwc);
         e->tw4.resize(n);
         e->itw4.resize(n);
@@ -108,6 +126,8 @@ extern "C" void *emu_create2(uint32_t p, uint32_t n, int prp, uint32_t lc, uint3
     P.itwr2 = gv<const u64>(e->itwr2);
     P.twc = gv<const u64>(e->twc);
     P.itwc = gv<const u64>(e->itwc);
+    P.twc2 = gv<const u64>(e->twc2);
+    P.itwc2 = gv<const u64>(e->itwc2);
     P.p = p;
     P.n = n;
     P.log_n = pl.log_n;
@@ -119,6 +139,10 @@ extern "C" void *emu_create2(uint32_t p, uint32_t n, int prp, uint32_t lc, uint3
     to_device_digits(e->wid, d.data(), e->x.data());
     e->pending = false;
     return e;
+}
+
+extern "C" void *emu_create2(uint32_t p, uint32_t n, int prp, uint32_t lc, uint32_t K, int row3) {
+    return emu_create3(p, n, prp, lc, K, row3, -1);
 }
 
 extern "C" void *emu_create(uint32_t p, uint32_t n, int prp, uint32_t force_lc) {
@@ -158,6 +182,22 @@ static void col_inv(Emu *e, const LntParams &P) {
     uint32_t lds = B1 * (B2 + LNT_PAD) * K;
     for (uint32_t b = 0; b < pl.C / K; b++)
         run_wg(K * B2, lds, [&](int tid, LView L) { lnt_col_inv_body<B1, B2, K>(tid, b, L, P); });
+}
+
+template <int BA, int BB, int BC, int K, int M>
+static void col3_fwd(Emu *e, bool sub2, const LntParams &P) {
+    const Plan &pl = e->pl;
+    for (uint32_t b = 0; b < pl.C / K; b++)
+        run_wg(K * BB * BC, LntCol3<BA, BB, BC, K>::LDS,
+               [&](int tid, LView L) { lnt_col3_fwd_body<BA, BB, BC, K, M>(tid, b, L, P, sub2); });
+}
+
+template <int BA, int BB, int BC, int K>
+static void col3_inv(Emu *e, const LntParams &P) {
+    const Plan &pl = e->pl;
+    for (uint32_t b = 0; b < pl.C / K; b++)
+        run_wg(K * BB * BC, LntCol3<BA, BB, BC, K>::LDS,
+               [&](int tid, LView L) { lnt_col3_inv_body<BA, BB, BC, K>(tid, b, L, P); });
 }
 
 template <int B, int G>
@@ -214,9 +254,21 @@ static bool run_large_once(Emu *e, bool sub2, const LntParams &P, int mode = LNT
         if (pl.M == 4) col_fwd<b1, b2, k, 4>(e, sub2, P); else col_fwd<b1, b2, k, 8>(e, sub2, P); \
         ok = true;                                                               \
     }
-    CF(8, 8, 16) CF(16, 8, 16) CF(16, 16, 16) CF(32, 16, 16) CF(32, 32, 16) CF(32, 32, 8) CF(64, 32, 8) CF(64, 32, 4)
-    CF(64, 64, 4)
+    if (!pl.col3) {
+        CF(8, 8, 16) CF(16, 8, 16) CF(16, 16, 16) CF(32, 16, 16) CF(32, 32, 16) CF(32, 32, 8) CF(64, 32, 8)
+        CF(64, 32, 4) CF(64, 64, 4)
+    }
 #undef CF
+#define C3F(ba, bb, bc, k)                                                                    \
+    if (!ok && pl.col3 == ba && pl.cBB == bb && pl.cBC == bc && pl.K == k) {                  \
+        if (pl.M == 4) col3_fwd<ba, bb, bc, k, 4>(e, sub2, P); else col3_fwd<ba, bb, bc, k, 8>(e, sub2, P); \
+        ok = true;                                                                            \
+    }
+    if (pl.col3) {
+        C3F(8, 8, 8, 16) C3F(8, 8, 8, 8) C3F(16, 8, 8, 16) C3F(16, 8, 8, 8) C3F(16, 16, 8, 8) C3F(16, 16, 8, 4)
+        C3F(16, 16, 16, 4)
+    }
+#undef C3F
     if (!ok) return false;
     ok = false;
     if (pl.row3 == 16) row3_sq<16, 1>(e, P, mode), ok = true;
@@ -236,9 +288,21 @@ static bool run_large_once(Emu *e, bool sub2, const LntParams &P, int mode = LNT
         col_inv<b1, b2, k>(e, P);                                               \
         ok = true;                                                              \
     }
-    CI(8, 8, 16) CI(16, 8, 16) CI(16, 16, 16) CI(32, 16, 16) CI(32, 32, 16) CI(32, 32, 8) CI(64, 32, 8) CI(64, 32, 4)
-    CI(64, 64, 4)
+    if (!pl.col3) {
+        CI(8, 8, 16) CI(16, 8, 16) CI(16, 16, 16) CI(32, 16, 16) CI(32, 32, 16) CI(32, 32, 8) CI(64, 32, 8)
+        CI(64, 32, 4) CI(64, 64, 4)
+    }
 #undef CI
+#define C3I(ba, bb, bc, k)                                                    \
+    if (!ok && pl.col3 == ba && pl.cBB == bb && pl.cBC == bc && pl.K == k) {  \
+        col3_inv<ba, bb, bc, k>(e, P);                                        \
+        ok = true;                                                            \
+    }
+    if (pl.col3) {
+        C3I(8, 8, 8, 16) C3I(8, 8, 8, 8) C3I(16, 8, 8, 16) C3I(16, 8, 8, 8) C3I(16, 16, 8, 8) C3I(16, 16, 8, 4)
+        C3I(16, 16, 16, 4)
+    }
+#undef C3I
     return ok;
 }
 

@@ -108,6 +108,8 @@ def emu():
         u32, vp = ctypes.c_uint32, ctypes.c_void_p
         L.emu_create.argtypes = [u32, u32, ctypes.c_int, u32]
         L.emu_create.restype = vp
+        L.emu_create3.argtypes = [u32, u32, ctypes.c_int, u32, u32, ctypes.c_int, ctypes.c_int]
+        L.emu_create3.restype = vp
         L.emu_destroy.argtypes = [vp]
         L.emu_iterate.argtypes = [vp, u32]
         L.emu_get_digits.argtypes = [vp, vp]

@@ -12,10 +12,10 @@ os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
 
 
 class Emu:
-    def __init__(self, p, n, prp=False, force_lc=0):
+    def __init__(self, p, n, prp=False, force_lc=0, K=0, row3=-1, col3=-1):
         self.L = emu()
         self.n = n
-        self.h = self.L.emu_create(p, n, int(prp), force_lc)
+        self.h = self.L.emu_create3(p, n, int(prp), force_lc, K, row3, col3)
         assert self.h, (p, n)
 
     def __del__(self):
@@ -60,6 +60,26 @@ def test_emulated_kernels_match_oracle_random_state(p, n, lc, prp):
     k = 3 if n >= 32768 else 6
     o.iterate(k)
     e.iterate(k)
+    assert e.digits() == o.digits()
+
+
+COL3_CASES = [  # (p, n, lc, K): 3-phase columns, R = 512, 1024, 2048, 4096
+    (1000003, 65536, 7, 8), (1000003, 65536, 7, 16), (2000003, 131072, 7, 8), (2000003, 131072, 7, 16),
+    (4000037, 262144, 7, 4), (4000037, 262144, 7, 8), (8000009, 524288, 7, 4),
+]
+
+
+@pytest.mark.parametrize("p,n,lc,K", COL3_CASES)
+def test_emulated_3phase_columns_match_oracle(p, n, lc, K):
+    rnd = random.Random(p + K)
+    prp = K == 16
+    o = OracleTest(p, n, prp)
+    d = [rnd.getrandbits(wi) for wi in o.widths()]
+    o.set_digits(d)
+    e = Emu(p, n, prp, lc, K, -1, 1)
+    e.set_digits(d)
+    o.iterate(2)
+    e.iterate(2)
     assert e.digits() == o.digits()
 
 
