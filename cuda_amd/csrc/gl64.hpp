@@ -29,17 +29,37 @@ typedef uint32_t u32;
 // a + b mod P.  Requires a + b < 2^64 + P - 2^32 (true for a < 2^64, b <= P - 2^32,
 // or a, b <= P).  The sum needs correcting iff it carried out of 64 bits or s + EPS does.
 GL_HD u64 gl_add(u64 a, u64 b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    u32 c0, c1, c2, c3;
+    const u32 sl = __builtin_addc((u32)a, (u32)b, 0u, &c0);
+    const u32 sh = __builtin_addc((u32)(a >> 32), (u32)(b >> 32), c0, &c1);
+    const u32 tl = __builtin_addc(sl, 0xFFFFFFFFu, 0u, &c2);
+    const u32 th = __builtin_addc(sh, 0u, c2, &c3);
+    return (c1 | c3) ? (((u64)th << 32) | tl) : (((u64)sh << 32) | sl);
+#else
     u64 s = a + b;
     bool c1 = s < a;
     u64 t = s + GL_EPS;
     bool c2 = t < s;
     return (c1 | c2) ? t : s;
+#endif
 }
 
 // a - b mod P for a < 2^64, b <= P: on borrow, a - b + 2^64 - EPS = a - b + P.
+// On the device the borrow of the 32-bit subtract chain selects the correction directly
+// (v_sub_co / v_subb, then subtract 0:EPS), one instruction fewer than a separate compare.
 GL_HD u64 gl_sub(u64 a, u64 b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    u32 b0, b1, b2, b3;
+    const u32 dl = __builtin_subc((u32)a, (u32)b, 0u, &b0);
+    const u32 dh = __builtin_subc((u32)(a >> 32), (u32)(b >> 32), b0, &b1);
+    const u32 rl = __builtin_subc(dl, b1 ? 0xFFFFFFFFu : 0u, 0u, &b2);
+    const u32 rh = __builtin_subc(dh, 0u, b2, &b3);
+    return ((u64)rh << 32) | rl;
+#else
     u64 d = a - b;
     return (a < b) ? d - GL_EPS : d;
+#endif
 }
 
 // -a mod P, canonical.
@@ -49,8 +69,7 @@ GL_HD u64 gl_neg(u64 a) { return a ? GL_P - a : 0; }
 // x = x1:x0 + x2 * (2^32 - 1) - x3 (mod P).  Result canonical for any hi, lo.
 GL_HD u64 gl_reduce128(u64 hi, u64 lo) {
     u32 x2 = (u32)hi, x3 = (u32)(hi >> 32);
-    u64 t0 = lo - x3;
-    if (lo < (u64)x3) t0 -= GL_EPS;  // lo - x3 + P, cannot underflow
+    u64 t0 = gl_sub(lo, x3);         // lo - x3 + P on borrow, cannot underflow
     u64 t1 = ((u64)x2 << 32) - x2;   // x2 * EPS <= P - 2^32
     return gl_add(t0, t1);
 }
