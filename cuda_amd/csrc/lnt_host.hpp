@@ -155,15 +155,17 @@ inline std::string make_plan(uint32_t p, uint64_t n, Plan &pl, PlanOpts opts = P
     pl.row = line_for(lc);
     pl.col = line_for(lr);
     pl.K = pl.R >= 4096 ? 4 : pl.R >= 2048 ? 8 : 16;
-    // 3-phase columns by default for R >= 2048 (measured faster there; equal at R = 1024)
-    const int col3 = opts.col3 >= 0 ? opts.col3 : (pl.R >= 2048 ? 1 : 0);
+    // 3-phase columns by default for R >= 1024 (measured faster there)
+    const int col3 = opts.col3 >= 0 ? opts.col3 : (pl.R >= 1024 ? 1 : 0);
     if (col3 == 1) {
         if (pl.R < 512 || pl.R > 4096) return "3-phase columns need R = 512 .. 4096";
         pl.col3 = pl.R >= 1024 ? 16 : 8;
         const uint32_t A = pl.R / pl.col3;
         pl.cBB = 1u << ((ilog2(A) + 1) / 2);
         pl.cBC = A / pl.cBB;
-        pl.K = pl.R >= 4096 ? 4 : 8;  // R = 2048: K = 8 (1024-thread blocks) measured faster than K = 4
+        // 1024-thread blocks: K = 16 at R = 1024, 8 at R = 2048, 4 at R = 4096 (measured
+        // faster than the 512-thread tiles)
+        pl.K = pl.R >= 4096 ? 4 : pl.R >= 2048 ? 8 : pl.R >= 1024 ? 16 : 8;
     }
     if (opts.K) {
         if (!(pl.col3 ? col3_tile_ok(pl.R, opts.K) : col_tile_ok(pl.R, opts.K))) return "unsupported column tile";

@@ -171,16 +171,10 @@ LNT_KFN void lnt_col_fwd_body(int tid, int bid, LView lds, const LntParams &P, b
     }
     LNT_SYNC();
 
-    // 2. phase 1: weight, B1-point DFT down the column, inter-block twiddle.  The 4-step
-    //    twiddles of phase 3 are fetched here so their latency hides behind the DFTs.
+    // 2. phase 1: weight, B1-point DFT down the column, inter-block twiddle
 #pragma unroll
     for (int j = 0; j < B1; j++)
         a[j] = gl_mul(lds[(u32)((t + B2 * j) * KS + cl)], PF ? wv[PF ? j : 0] : P.w[c + C * (u64)(t + B2 * j)]);
-    u64 tv[PF ? B1 : 1];
-    if (PF) {
-#pragma unroll
-        for (int i = 0; i < B1; i++) tv[PF ? i : 0] = P.tw4[c + C * ((u64)t * B1 + i)];
-    }
     dft_fwd<B1>(a);
     if (B2 > 1) {
 #pragma unroll
@@ -192,7 +186,8 @@ LNT_KFN void lnt_col_fwd_body(int tid, int bid, LView lds, const LntParams &P, b
         lds[(u32)((brev_c<B1>(rho) * S + t) * K + cl)] = a[rho];
     LNT_SYNC();
 
-    // 3. phase 2: B2-point DFTs, 4-step twiddle, store (row r' = t*B1 + q*B2 + rho')
+    // 3. phase 2: B2-point DFTs, store (row r' = t*B1 + q*B2 + rho'); the 4-step twiddle
+    //    is applied by the row pass as it loads the row
 #pragma unroll
     for (int q = 0; q < RP; q++) {
         const int k2 = t * RP + q;
@@ -201,10 +196,7 @@ LNT_KFN void lnt_col_fwd_body(int tid, int bid, LView lds, const LntParams &P, b
         dft_fwd<B2>(a + q * B2);
     }
 #pragma unroll
-    for (int i = 0; i < B1; i++) {
-        const u64 idx = c + C * ((u64)t * B1 + i);
-        P.y[idx] = gl_mul(a[i], PF ? tv[PF ? i : 0] : P.tw4[idx]);
-    }
+    for (int i = 0; i < B1; i++) P.y[c + C * ((u64)t * B1 + i)] = a[i];
 }
 
 // ------------------------------------------------------------------ column inverse
@@ -217,11 +209,9 @@ LNT_KFN void lnt_col_inv_body(int tid, int bid, LView lds, const LntParams &P) {
     const u64 c = (u64)bid * K + cl;
     constexpr bool PF = B1 <= 32;  // prefetch the unweighting factors (fits when B1 <= 32)
     u64 a[B1], uv[PF ? B1 : 1];
+    // (the inverse 4-step twiddle was applied by the row pass as it stored the row)
 #pragma unroll
-    for (int i = 0; i < B1; i++) {
-        const u64 idx = c + C * ((u64)t * B1 + i);
-        a[i] = gl_mul(P.y[idx], P.itw4[idx]);
-    }
+    for (int i = 0; i < B1; i++) a[i] = P.y[c + C * ((u64)t * B1 + i)];
     if (PF) {
 #pragma unroll
         for (int j = 0; j < B1; j++) uv[PF ? j : 0] = P.uw[c + C * ((u64)t + (u64)B2 * j)];
@@ -309,15 +299,26 @@ LNT_KFN void lnt_col3_fwd_body(int tid, int bid, LView lds, const LntParams &P, 
     for (int q = 0; q < RT; q++) {
         const int r = tid + NT * q;
         if (r >= RR) break;
-        int64_t v[M + K];
+        // streamed in pieces of PK digits, each piece's halo being the M pre-carry values
+        // before it (blocks of 1024 threads keep the run within their 128 registers)
+        constexpr int PK = (NT >= 1024 && K > 8) ? 8 : K;
         const u64 g0 = C * (u64)r + c0;
+        int64_t halo[M];
 #pragma unroll
-        for (int j = 0; j < M; j++) v[j] = gl_to_i64(HP ? h[HP ? q : 0][j] : P.x[(g0 + n - M + j) & (n - 1)]);
+        for (int j = 0; j < M; j++) halo[j] = gl_to_i64(HP ? h[HP ? q : 0][j] : P.x[(g0 + n - M + j) & (n - 1)]);
 #pragma unroll
-        for (int j = 0; j < K; j++) v[M + j] = gl_to_i64(lds[(u32)(r * KS + j)]);
-        lnt_carry_run<M, K>(v, P.p, P.log_n, g0, P.levels, sub2);
+        for (int s = 0; s < K / PK; s++) {
+            int64_t v[M + PK];
 #pragma unroll
-        for (int j = 0; j < K; j++) lds[(u32)(r * KS + j)] = gl_from_i64(v[M + j]);
+            for (int j = 0; j < M; j++) v[j] = halo[j];
+#pragma unroll
+            for (int j = 0; j < PK; j++) v[M + j] = gl_to_i64(lds[(u32)(r * KS + s * PK + j)]);
+#pragma unroll
+            for (int j = 0; j < M; j++) halo[j] = v[PK + j];
+            lnt_carry_run<M, PK>(v, P.p, P.log_n, g0 + s * PK, P.levels, sub2);
+#pragma unroll
+            for (int j = 0; j < PK; j++) lds[(u32)(r * KS + s * PK + j)] = gl_from_i64(v[M + j]);
+        }
     }
     LNT_SYNC();
 
@@ -348,7 +349,7 @@ LNT_KFN void lnt_col3_fwd_body(int tid, int bid, LView lds, const LntParams &P, 
     }
     LNT_SYNC();
 
-    // 4. phase 3, 4-step twiddle, store slot t*BA + q*BC + rho
+    // 4. phase 3, store slot t*BA + q*BC + rho (the row pass applies the 4-step twiddle)
 #pragma unroll
     for (int q = 0; q < S::QC; q++) {
         const int I = q * A + t, k1 = I / BB, k2 = I % BB;
@@ -357,10 +358,7 @@ LNT_KFN void lnt_col3_fwd_body(int tid, int bid, LView lds, const LntParams &P, 
         dft_fwd<BC>(a + q * BC);
     }
 #pragma unroll
-    for (int i = 0; i < BA; i++) {
-        const u64 idx = c + C * ((u64)t * BA + i);
-        P.y[idx] = gl_mul(a[i], P.tw4[idx]);
-    }
+    for (int i = 0; i < BA; i++) P.y[c + C * ((u64)t * BA + i)] = a[i];
 }
 
 template <int BA, int BB, int BC, int K>
@@ -372,10 +370,7 @@ LNT_KFN void lnt_col3_inv_body(int tid, int bid, LView lds, const LntParams &P) 
     const u64 c = (u64)bid * K + cl;
     u64 a[BA], uv[BA];
 #pragma unroll
-    for (int i = 0; i < BA; i++) {
-        const u64 idx = c + C * ((u64)t * BA + i);
-        a[i] = gl_mul(P.y[idx], P.itw4[idx]);
-    }
+    for (int i = 0; i < BA; i++) a[i] = P.y[c + C * ((u64)t * BA + i)];
 #pragma unroll
     for (int j = 0; j < BA; j++) uv[j] = P.uw[c + C * ((u64)t + (u64)A * j)];
     // phase 3 inverse
@@ -433,7 +428,10 @@ LNT_KFN void lnt_row_body(int tid, int bid, LView lds, const LntParams &P) {
     const u32 lb = (u32)(g * LINE);
     u64 a[B1];
 #pragma unroll
-    for (int j = 0; j < B1; j++) a[j] = P.y[base + t + (u64)B2 * j];
+    for (int j = 0; j < B1; j++) {
+        const u64 idx = base + t + (u64)B2 * j;
+        a[j] = gl_mul(P.y[idx], P.tw4[idx]);  // 4-step twiddle of the column pass
+    }
     dft_fwd<B1>(a);
     if (B2 > 1) {
 #pragma unroll
@@ -474,7 +472,10 @@ LNT_KFN void lnt_row_body(int tid, int bid, LView lds, const LntParams &P) {
     }
     dft_inv<B1>(a);
 #pragma unroll
-    for (int j = 0; j < B1; j++) P.y[base + t + (u64)B2 * j] = a[j];
+    for (int j = 0; j < B1; j++) {
+        const u64 idx = base + t + (u64)B2 * j;
+        P.y[idx] = gl_mul(a[j], P.itw4[idx]);  // inverse 4-step twiddle
+    }
 }
 
 // ------------------------------------------------------------------ 3-phase row pass
@@ -500,7 +501,17 @@ LNT_KFN void lnt_row3_body(int tid, int bid, LView lds, const LntParams &P) {
     u64 a[B];
     // phase 1
 #pragma unroll
-    for (int j = 0; j < B; j++) a[j] = P.y[base + t + (u64)T * j];
+    for (int j = 0; j < B; j++) {
+        const u64 idx = base + t + (u64)T * j;
+        a[j] = gl_mul(P.y[idx], P.tw4[idx]);  // 4-step twiddle of the column pass
+    }
+    // the inverse 4-step twiddles of the final store, fetched now so their latency hides
+    // behind the transforms
+    u64 iv[MODE == LNT_FORWARD ? 1 : B];
+    if (MODE != LNT_FORWARD) {
+#pragma unroll
+        for (int j = 0; j < B; j++) iv[MODE == LNT_FORWARD ? 0 : j] = P.itw4[base + t + (u64)T * j];
+    }
     dft_fwd<B>(a);
 #pragma unroll
     for (int rho = 1; rho < B; rho++) a[rho] = gl_mul(a[rho], P.twr[(u64)rho * T + t]);
@@ -555,7 +566,7 @@ LNT_KFN void lnt_row3_body(int tid, int bid, LView lds, const LntParams &P) {
     }
     dft_inv<B>(a);
 #pragma unroll
-    for (int j = 0; j < B; j++) P.y[base + t + (u64)T * j] = a[j];
+    for (int j = 0; j < B; j++) P.y[base + t + (u64)T * j] = gl_mul(a[j], iv[MODE == LNT_FORWARD ? 0 : j]);
 }
 
 // ------------------------------------------------------------------ small residues
