@@ -94,7 +94,9 @@ inline uint32_t carry_levels(uint32_t p, uint32_t n) {
 // Row/column split of a large residue: log2 C and log2 R.
 inline bool split_for(uint32_t log_n, uint32_t &lc, uint32_t &lr) {
     if (log_n < 13 || log_n > 24) return false;
-    lc = log_n >= 22 ? 12 : (log_n + 1) / 2;
+    // rows of 4096 words (3-phase row kernel) from n = 2^18 on: measured 1.5-2x faster than
+    // the square-ish splits with 2-phase rows of 1024 or 2048 words
+    lc = log_n >= 18 ? 12 : (log_n + 1) / 2;
     lr = log_n - lc;
     return lr <= 12 && lr >= 6;
 }
@@ -155,8 +157,8 @@ inline std::string make_plan(uint32_t p, uint64_t n, Plan &pl, PlanOpts opts = P
     pl.row = line_for(lc);
     pl.col = line_for(lr);
     pl.K = pl.R >= 4096 ? 4 : pl.R >= 2048 ? 8 : 16;
-    // 3-phase columns by default for R >= 1024 (measured faster there)
-    const int col3 = opts.col3 >= 0 ? opts.col3 : (pl.R >= 1024 ? 1 : 0);
+    // 3-phase columns by default for R >= 512 (measured faster there)
+    const int col3 = opts.col3 >= 0 ? opts.col3 : (pl.R >= 512 ? 1 : 0);
     if (col3 == 1) {
         if (pl.R < 512 || pl.R > 4096) return "3-phase columns need R = 512 .. 4096";
         pl.col3 = pl.R >= 1024 ? 16 : 8;
