@@ -68,13 +68,15 @@ __global__ __launch_bounds__(G * B * B) void lnt_k_row3(LntParams P) {
 
 // ---- batched launches: many tests of one shape in one grid.  Workgroup b serves test
 // b / wgs (its parameters Ps[b / wgs], uniform per workgroup) as that test's workgroup
-// b % wgs.  The "-2" flag is per test (Ps[].sub2).
+// b % wgs.  The "-2" flag is per test (Ps[].sub2).  Each kernel takes a private copy of
+// its parameter block: read through a reference into global memory, every field would
+// have to be re-read after each global store (possible aliasing), which costs registers.
 template <int B1, int B2, int M>
 __global__ __launch_bounds__(B2 < 64 ? 64 : B2) void lnt_kb_small(const LntParams *__restrict__ Ps, u32 iters,
                                                                    int ll) {
     constexpr u32 W = (B1 * B2 > B1 * (B2 + LNT_PAD)) ? B1 * B2 : B1 * (B2 + LNT_PAD);
     __shared__ u64 smem[W];
-    const LntParams &P = Ps[blockIdx.x];
+    const LntParams P = Ps[blockIdx.x];
     lnt_small_body<B1, B2, M>(threadIdx.x, LView{smem, W}, P, iters, P.sub2 != 0, ll != 0);
 }
 
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(K * B2) void lnt_kb_col_fwd(const LntParams *__rest
     constexpr u32 W = lnt_col_fwd_lds<B1, B2, K>();
     __shared__ u64 smem[W];
     const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
-    const LntParams &P = Ps[b / wgs];
+    const LntParams P = Ps[b / wgs];
     lnt_col_fwd_body<B1, B2, K, M>(threadIdx.x, b % wgs, LView{smem, W}, P, P.sub2 != 0);
 }
 
@@ -92,7 +94,8 @@ __global__ __launch_bounds__(K * B2) void lnt_kb_col_inv(const LntParams *__rest
     constexpr u32 W = B1 * (B2 + LNT_PAD) * K;
     __shared__ u64 smem[W];
     const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
-    lnt_col_inv_body<B1, B2, K>(threadIdx.x, b % wgs, LView{smem, W}, Ps[b / wgs]);
+    const LntParams P = Ps[b / wgs];
+    lnt_col_inv_body<B1, B2, K>(threadIdx.x, b % wgs, LView{smem, W}, P);
 }
 
 template <int BA, int BB, int BC, int K, int M>
@@ -100,7 +103,7 @@ __global__ __launch_bounds__(K * BB * BC) void lnt_kb_col3_fwd(const LntParams *
     constexpr u32 W = LntCol3<BA, BB, BC, K>::LDS;
     __shared__ u64 smem[W];
     const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
-    const LntParams &P = Ps[b / wgs];
+    const LntParams P = Ps[b / wgs];
     lnt_col3_fwd_body<BA, BB, BC, K, M>(threadIdx.x, b % wgs, LView{smem, W}, P, P.sub2 != 0);
 }
 
@@ -109,21 +112,24 @@ __global__ __launch_bounds__(K * BB * BC) void lnt_kb_col3_inv(const LntParams *
     constexpr u32 W = LntCol3<BA, BB, BC, K>::LDS;
     __shared__ u64 smem[W];
     const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
-    lnt_col3_inv_body<BA, BB, BC, K>(threadIdx.x, b % wgs, LView{smem, W}, Ps[b / wgs]);
+    const LntParams P = Ps[b / wgs];
+    lnt_col3_inv_body<BA, BB, BC, K>(threadIdx.x, b % wgs, LView{smem, W}, P);
 }
 
 template <int B1, int B2, int G>
 __global__ __launch_bounds__(G * B2) void lnt_kb_row(const LntParams *__restrict__ Ps, u32 wgs) {
     constexpr u32 W = G * B1 * (B2 + LNT_PAD);
     __shared__ u64 smem[W];
-    lnt_row_body<B1, B2, G>(threadIdx.x, blockIdx.x % wgs, LView{smem, W}, Ps[blockIdx.x / wgs]);
+    const LntParams P = Ps[blockIdx.x / wgs];
+    lnt_row_body<B1, B2, G>(threadIdx.x, blockIdx.x % wgs, LView{smem, W}, P);
 }
 
 template <int B, int G>
 __global__ __launch_bounds__(G * B * B) void lnt_kb_row3(const LntParams *__restrict__ Ps, u32 wgs) {
     constexpr u32 W = G * B * B * (B + 1);
     __shared__ u64 smem[W];
-    lnt_row3_body<B, G>(threadIdx.x, blockIdx.x % wgs, LView{smem, W}, Ps[blockIdx.x / wgs]);
+    const LntParams P = Ps[blockIdx.x / wgs];
+    lnt_row3_body<B, G>(threadIdx.x, blockIdx.x % wgs, LView{smem, W}, P);
 }
 
 // Batched launchers: `count` tests of plan pl, parameters in device memory.

@@ -30,14 +30,25 @@
 #endif
 
 // Global-memory and LDS views carrying their length so the emulation can check.
+// On the device every access goes through a global-address-space pointer: the batched
+// kernels read their views from a parameter array in memory, where the compiler cannot
+// infer the address space and would otherwise emit flat accesses with a 64-bit VGPR
+// address per element (twice the registers of the saddr + offset global form).
 template <class T>
 struct GView {
     T *p;
     u64 n;
+#if defined(__HIP_DEVICE_COMPILE__)
+    __device__ __forceinline__ T &operator[](u64 i) const {
+        typedef __attribute__((address_space(1))) T GT;
+        return *(T *)((GT *)p + i);
+    }
+#else
     GL_HDM T &operator[](u64 i) const {
         LNT_CHECK(i < n, "global index");
         return p[i];
     }
+#endif
 };
 
 struct LView {
