@@ -505,7 +505,9 @@ uint32_t lnt_length(const lnt_ctx *c) { return c ? c->pl.n : 0; }
 
 static std::string describe(const Plan &pl) {
     char b[320];
-    if (pl.small) {
+    if (pl.small && pl.row3) {
+        std::snprintf(b, sizeof b, "LNT_1WG;lnt_k_small3<%u>", pl.row3);
+    } else if (pl.small) {
         std::snprintf(b, sizeof b, "LNT_1WG;lnt_k_small<%u,%u>", pl.row.B1, pl.row.B2);
     } else {
         char row[64];

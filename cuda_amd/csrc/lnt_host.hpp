@@ -143,6 +143,8 @@ inline std::string make_plan(uint32_t p, uint64_t n, Plan &pl, PlanOpts opts = P
         pl.row = line_for(pl.log_n);
         pl.C = pl.n;
         pl.R = 1;
+        // 3-phase single-workgroup kernel at n = 4096 = 16^3 and 512 = 8^3 unless asked not to
+        if (opts.row3 != 0 && (pl.n == 4096 || pl.n == 512)) pl.row3 = pl.n == 4096 ? 16 : 8;
         return "";
     }
     uint32_t lc, lr;

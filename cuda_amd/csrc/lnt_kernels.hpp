@@ -674,6 +674,120 @@ LNT_KFN void lnt_small_body(int tid, LView lds, const LntParams &P, u32 iters, b
     for (int j = 0; j < B1; j++) P.x[(u64)t + B2 * j] = lds[(u32)(t + B2 * j)];
 }
 
+// ------------------------------------------------------------------ small residues, 3 phases
+// One workgroup of B^2 threads runs `iters` iterations on n = B^3 words (B = 16: 4096,
+// B = 8: 512), B words per thread in every phase (four waves at n = 4096 where the
+// 2-phase kernel runs one).  LDS: the residue in natural order (n words), then the
+// transform region (B * B * (B + 1) words, the 3-phase row layout).  MODEs as above.
+template <int B>
+constexpr u32 lnt_small3_lds() { return (u32)(B * B * B + B * B * (B + 1)); }
+
+template <int B, int M, int MODE = LNT_SQUARE>
+LNT_KFN void lnt_small3_body(int tid, LView lds, const LntParams &P, u32 iters, bool sub2_first, bool ll) {
+    constexpr int T = B * B;
+    constexpr u32 N = B * B * B;
+    constexpr int RS = B * (B + 1);
+    constexpr u32 Z = N;  // transform region
+    const int t = tid, hi = t / B, lo = t % B;
+#pragma unroll
+    for (int j = 0; j < B; j++) lds[(u32)(t + T * j)] = P.x[(u64)t + T * j];
+    LNT_SYNC();
+    for (u32 it = 0; it < iters; it++) {
+        // tables re-read each iteration (from cache) instead of living in registers
+        const u64 *pw = P.w.p, *puw = P.uw.p, *ptw = P.twr.p, *pitw = P.itwr.p, *ptw2 = P.twr2.p,
+                  *pitw2 = P.itwr2.p;
+        LNT_OPAQUE(pw);
+        LNT_OPAQUE(puw);
+        LNT_OPAQUE(ptw);
+        LNT_OPAQUE(pitw);
+        LNT_OPAQUE(ptw2);
+        LNT_OPAQUE(pitw2);
+        const GView<const u64> Tw{pw, P.w.n}, Tuw{puw, P.uw.n}, Ttw{ptw, P.twr.n}, Titw{pitw, P.itwr.n},
+            Ttw2{ptw2, P.twr2.n}, Titw2{pitw2, P.itwr2.n};
+        // carry over this thread's chunk [t*B, (t+1)*B); halo = the M pre-carry words before it
+        {
+            const u32 g0 = (u32)t * B;
+            int64_t v[M + B];
+#pragma unroll
+            for (int j = 0; j < M; j++) v[j] = gl_to_i64(lds[(g0 + N - M + j) & (N - 1)]);
+#pragma unroll
+            for (int j = 0; j < B; j++) v[M + j] = gl_to_i64(lds[g0 + j]);
+            LNT_SYNC();  // every halo read before any chunk is overwritten
+            lnt_carry_run<M, B>(v, P.p, P.log_n, g0, P.levels, it == 0 ? sub2_first : ll);
+#pragma unroll
+            for (int j = 0; j < B; j++) lds[g0 + j] = gl_from_i64(v[M + j]);
+            LNT_SYNC();
+        }
+        u64 a[B];
+        // phase 1: weight, B-point DFT over j (stride T), twiddle w_n^(t k1)
+#pragma unroll
+        for (int j = 0; j < B; j++) {
+            const u32 i = (u32)(t + T * j);
+            a[j] = gl_mul(lds[i], Tw[i]);
+        }
+        dft_fwd<B>(a);
+#pragma unroll
+        for (int rho = 1; rho < B; rho++) a[rho] = gl_mul(a[rho], Ttw[(u64)rho * T + t]);
+#pragma unroll
+        for (int rho = 0; rho < B; rho++) lds[Z + (u32)(brev_c<B>(rho) * RS + hi * (B + 1) + lo)] = a[rho];
+        LNT_SYNC();
+        // phase 2: thread (hi, lo) over the middle index
+#pragma unroll
+        for (int b = 0; b < B; b++) a[b] = lds[Z + (u32)(hi * RS + b * (B + 1) + lo)];
+        dft_fwd<B>(a);
+#pragma unroll
+        for (int rho = 1; rho < B; rho++) a[rho] = gl_mul(a[rho], Ttw2[(u64)rho * B + lo]);
+#pragma unroll
+        for (int rho = 0; rho < B; rho++) lds[Z + (u32)(hi * RS + brev_c<B>(rho) * (B + 1) + lo)] = a[rho];
+        LNT_SYNC();
+        // phase 3, pointwise square (or product), inverse phase 3
+#pragma unroll
+        for (int i = 0; i < B; i++) a[i] = lds[Z + (u32)(hi * RS + lo * (B + 1) + i)];
+        dft_fwd<B>(a);
+        if (MODE == LNT_FORWARD) {
+#pragma unroll
+            for (int i = 0; i < B; i++) P.y[(u64)t * B + i] = a[i];
+            return;
+        }
+        if (MODE == LNT_MULTIPLY) {
+#pragma unroll
+            for (int i = 0; i < B; i++) a[i] = gl_mul(a[i], P.f[(u64)t * B + i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < B; i++) a[i] = gl_sqr(a[i]);
+        }
+        dft_inv<B>(a);
+#pragma unroll
+        for (int i = 0; i < B; i++) lds[Z + (u32)(hi * RS + lo * (B + 1) + i)] = a[i];
+        LNT_SYNC();
+        // inverse phase 2
+#pragma unroll
+        for (int rho = 0; rho < B; rho++) {
+            u64 v = lds[Z + (u32)(hi * RS + brev_c<B>(rho) * (B + 1) + lo)];
+            a[rho] = rho ? gl_mul(v, Titw2[(u64)rho * B + lo]) : v;
+        }
+        dft_inv<B>(a);
+#pragma unroll
+        for (int b = 0; b < B; b++) lds[Z + (u32)(hi * RS + b * (B + 1) + lo)] = a[b];
+        LNT_SYNC();
+        // inverse phase 1, unweight, back to the residue region
+#pragma unroll
+        for (int rho = 0; rho < B; rho++) {
+            u64 v = lds[Z + (u32)(brev_c<B>(rho) * RS + hi * (B + 1) + lo)];
+            a[rho] = rho ? gl_mul(v, Titw[(u64)rho * T + t]) : v;
+        }
+        dft_inv<B>(a);
+#pragma unroll
+        for (int j = 0; j < B; j++) {
+            const u32 i = (u32)(t + T * j);
+            lds[i] = gl_mul(a[j], Tuw[i]);
+        }
+        LNT_SYNC();
+    }
+#pragma unroll
+    for (int j = 0; j < B; j++) P.x[(u64)t + T * j] = lds[(u32)(t + T * j)];
+}
+
 // ------------------------------------------------------------------ tables
 // Twiddles of a line of L = B1 * B2 words: tw[rho * B2 + t] = omega_L^(t * brev_B1(rho)).
 GL_HD u64 lnt_line_tw(u64 omega_L, u32 B1, u32 B2, u32 idx) {

@@ -23,6 +23,13 @@ __global__ __launch_bounds__(B2 < 64 ? 64 : B2) void lnt_k_small(LntParams P, u3
     lnt_small_body<B1, B2, M, MODE>(threadIdx.x, LView{smem, W}, P, iters, sub2_first != 0, ll != 0);
 }
 
+template <int B, int M, int MODE>
+__global__ __launch_bounds__(B * B) void lnt_k_small3(LntParams P, u32 iters, int sub2_first, int ll) {
+    constexpr u32 W = lnt_small3_lds<B>();
+    __shared__ u64 smem[W];
+    lnt_small3_body<B, M, MODE>(threadIdx.x, LView{smem, W}, P, iters, sub2_first != 0, ll != 0);
+}
+
 template <int B1, int B2, int K, int M>
 __global__ __launch_bounds__(K * B2) void lnt_k_col_fwd(LntParams P, int sub2) {
     constexpr u32 W = lnt_col_fwd_lds<B1, B2, K>();
@@ -78,6 +85,14 @@ __global__ __launch_bounds__(B2 < 64 ? 64 : B2) void lnt_kb_small(const LntParam
     __shared__ u64 smem[W];
     const LntParams P = Ps[blockIdx.x];
     lnt_small_body<B1, B2, M>(threadIdx.x, LView{smem, W}, P, iters, P.sub2 != 0, ll != 0);
+}
+
+template <int B, int M>
+__global__ __launch_bounds__(B * B) void lnt_kb_small3(const LntParams *__restrict__ Ps, u32 iters, int ll) {
+    constexpr u32 W = lnt_small3_lds<B>();
+    __shared__ u64 smem[W];
+    const LntParams P = Ps[blockIdx.x];
+    lnt_small3_body<B, M>(threadIdx.x, LView{smem, W}, P, iters, P.sub2 != 0, ll != 0);
 }
 
 template <int B1, int B2, int K, int M>

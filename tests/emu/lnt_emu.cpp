@@ -168,6 +168,23 @@ static void small_iter(Emu *e, uint32_t k, const LntParams &P, bool sub2) {
     run_wg(B2, lds, [&](int tid, LView L) { lnt_small_body<B1, B2, M, MODE>(tid, L, P, k, sub2, !e->prp); });
 }
 
+template <int B, int M, int MODE = LNT_SQUARE>
+static void small3_iter(Emu *e, uint32_t k, const LntParams &P, bool sub2) {
+    run_wg(B * B, lnt_small3_lds<B>(), [&](int tid, LView L) { lnt_small3_body<B, M, MODE>(tid, L, P, k, sub2, !e->prp); });
+}
+
+template <int B>
+static void small3_any(Emu *e, uint32_t k, const LntParams &P, bool sub2, int mode) {
+    if (mode == LNT_FORWARD) {
+        if (e->pl.M == 4) small3_iter<B, 4, LNT_FORWARD>(e, 1, P, sub2);
+        else small3_iter<B, 8, LNT_FORWARD>(e, 1, P, sub2);
+    } else if (mode == LNT_MULTIPLY) {
+        if (e->pl.M == 4) small3_iter<B, 4, LNT_MULTIPLY>(e, 1, P, sub2);
+        else small3_iter<B, 8, LNT_MULTIPLY>(e, 1, P, sub2);
+    } else if (e->pl.M == 4) small3_iter<B, 4>(e, k, P, sub2);
+    else small3_iter<B, 8>(e, k, P, sub2);
+}
+
 template <int B1, int B2, int K, int M>
 static void col_fwd(Emu *e, bool sub2, const LntParams &P) {
     const Plan &pl = e->pl;
@@ -227,6 +244,8 @@ static void row_sq(Emu *e, const LntParams &P, int mode) {
 // Dispatch tables for the emulated sizes (a subset of the HIP build's).
 static bool run_small(Emu *e, uint32_t k, const LntParams &P, bool sub2, int mode = LNT_SQUARE) {
     const Plan &pl = e->pl;
+    if (pl.row3 == 16) return small3_any<16>(e, k, P, sub2, mode), true;
+    if (pl.row3 == 8) return small3_any<8>(e, k, P, sub2, mode), true;
 #define SM(b1, b2)                                                                          \
     if (pl.row.B1 == b1 && pl.row.B2 == b2) {                                               \
         if (mode == LNT_FORWARD) {                                                          \
