@@ -501,6 +501,71 @@ const char *lnt_last_error(const lnt_ctx *c) {
 
 uint32_t lnt_secured_log2_length(uint32_t p) { return secured_log2_length(p); }
 
+}  // extern "C"
+
+// Field-op self-test: the device forms of gl64.hpp (carry chains) against the host forms.
+__global__ void lnt_k_field_ops(const u64 *a, const u64 *b, u32 count, u64 *out) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const u64 x = a[i], y = b[i];
+    u64 *o = out + (u64)i * 8;
+    o[0] = gl_add(x, y);
+    o[1] = gl_sub(x, y);
+    o[2] = gl_mul(x, y);
+    o[3] = gl_sqr(x);
+    o[4] = gl_mul_pow2(x, 12);
+    o[5] = gl_mul_pow2(x, 48);
+    o[6] = gl_sub_pow2(x, y, 84);
+    o[7] = gl_sub_pow2(x, y, 132);
+}
+
+extern "C" {
+
+int lnt_selftest(int device, uint32_t count, uint64_t seed, uint64_t *mismatches) {
+    if (!mismatches || count == 0) return fail(nullptr, LNT_INVALID_ARGUMENT, "bad argument");
+    if (hipSetDevice(device) != hipSuccess) return fail(nullptr, LNT_DEVICE_ERROR, "hipSetDevice failed");
+    // operands: canonical values with the edge cases first (0, 1, P-1, 2^32, EPS, ...)
+    std::vector<u64> a(count), b(count), out((size_t)count * 8);
+    const u64 edge[] = {0, 1, 2, GL_P - 1, GL_P - 2, 1ull << 32, GL_EPS, (1ull << 63), GL_P >> 1, (GL_P >> 1) + 1};
+    const u32 ne = sizeof edge / sizeof edge[0];
+    u64 st = seed;
+    auto next = [&]() {  // splitmix64
+        st += 0x9E3779B97F4A7C15ull;
+        u64 z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    for (u32 i = 0; i < count; i++) {
+        a[i] = i < ne * ne ? edge[i / ne] : next() % GL_P;
+        b[i] = i < ne * ne ? edge[i % ne] : next() % GL_P;
+    }
+    u64 *da = nullptr, *db = nullptr, *dout = nullptr;
+    bool ok = hipMalloc(&da, (size_t)count * 8) == hipSuccess && hipMalloc(&db, (size_t)count * 8) == hipSuccess &&
+              hipMalloc(&dout, (size_t)count * 64) == hipSuccess &&
+              hipMemcpy(da, a.data(), (size_t)count * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(db, b.data(), (size_t)count * 8, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(lnt_k_field_ops, dim3((count + 255) / 256), dim3(256), 0, 0, da, db, count, dout);
+        ok = hipGetLastError() == hipSuccess &&
+             hipMemcpy(out.data(), dout, (size_t)count * 64, hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (da) (void)hipFree(da);
+    if (db) (void)hipFree(db);
+    if (dout) (void)hipFree(dout);
+    if (!ok) return fail(nullptr, LNT_DEVICE_ERROR, "lnt_selftest: device error");
+    u64 bad = 0;
+    for (u32 i = 0; i < count; i++) {
+        const u64 x = a[i], y = b[i], *o = &out[(size_t)i * 8];
+        const u64 want[8] = {gl_add(x, y),          gl_sub(x, y),          gl_mul(x, y),
+                             gl_sqr(x),             gl_mul_pow2(x, 12),    gl_mul_pow2(x, 48),
+                             gl_sub_pow2(x, y, 84), gl_sub_pow2(x, y, 132)};
+        for (int k = 0; k < 8; k++) bad += o[k] != want[k];
+    }
+    *mismatches = bad;
+    return LNT_OK;
+}
+
 uint32_t lnt_length(const lnt_ctx *c) { return c ? c->pl.n : 0; }
 
 static std::string describe(const Plan &pl) {

@@ -35,7 +35,7 @@ EXPORTS = (
     "lnt_batch_add", "lnt_batch_remove", "lnt_batch_size", "lnt_batch_iterate", "lnt_batch_sync",
     "lnt_batch_destroy", "lnt_batch_last_error", "lnt_server_create", "lnt_server_submit",
     "lnt_server_poll", "lnt_server_stats", "lnt_server_now", "lnt_server_destroy",
-    "lnt_server_last_error",
+    "lnt_server_last_error", "lnt_selftest",
 )
 
 
@@ -110,6 +110,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lnt_batch_destroy.restype = None
     lib.lnt_batch_last_error.argtypes = [vp]
     lib.lnt_batch_last_error.restype = ctypes.c_char_p
+    lib.lnt_selftest.argtypes = [ctypes.c_int, u32, ctypes.c_uint64, u64p]
     lib.lnt_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
     lib.lnt_server_submit.argtypes = [vp, u32, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64]
     lib.lnt_server_poll.argtypes = [vp, ctypes.POINTER(Result), ctypes.c_int, ctypes.c_int]
@@ -122,6 +123,16 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lnt_server_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
+
+
+def field_selftest(count: int = 1 << 16, seed: int = 1, device: int = 0) -> int:
+    """Device field ops against the host forms; returns the number of mismatches."""
+    lib = load()
+    bad = ctypes.c_uint64()
+    rc = lib.lnt_selftest(device, count, seed, ctypes.byref(bad))
+    if rc != LNT_OK:
+        raise LntError(rc, lib.lnt_last_error(None).decode())
+    return int(bad.value)
 
 
 def secured_log2_length(p: int) -> int:

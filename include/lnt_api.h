@@ -81,6 +81,11 @@ const char *lnt_last_error(const lnt_ctx *ctx);
  * (NttLengths::GetNttSecuredLength, ntt_lengths.cu:88-98), in exact integer arithmetic. */
 uint32_t lnt_secured_log2_length(uint32_t p);
 
+/* Self-test of the device field arithmetic (new): add, sub, mul, square and the shift
+ * multiplies computed on the GPU for `count` operand pairs (edge values, then splitmix64
+ * draws from `seed`) and compared with the host forms; *mismatches = differing results. */
+int lnt_selftest(int device, uint32_t count, uint64_t seed, uint64_t *mismatches);
+
 /* Number of digits n of a context, and its description: "name;kernel0;kernel1;..." used
  * for the stats file's Pool / Kernel<i> keys (lucas_prp.cu:311-313). */
 uint32_t lnt_length(const lnt_ctx *ctx);

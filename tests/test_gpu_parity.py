@@ -139,3 +139,10 @@ def test_tune_keeps_residue():
     u.iterate(50)
     assert t.res64() == u.res64()
     assert t.get_digits() == u.get_digits()
+
+
+def test_device_field_ops_equal_host_forms():
+    """The device forms of the field arithmetic (32-bit carry chains) against the host
+    forms, which the oracle tests pin to Python integers mod P."""
+    from cuda_amd.lnt import field_selftest
+    assert field_selftest(1 << 18, seed=7) == 0
