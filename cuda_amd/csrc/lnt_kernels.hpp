@@ -12,16 +12,18 @@
 //            then the inter-block twiddle omega_L^(t * k2);
 //   phase 2: after an LDS transpose, B2-point register DFTs over t.
 //   Output X[B1*k1 + k2].  All roots of order <= 64 are powers of two, so the only
-//   general twiddle inside a line is the one between the phases.
+//   general twiddle inside a line is the one between the phases.  3-phase lines
+//   (rows of 16^3 words, columns of BA * BB * BC words) add one more phase and twiddle.
 // Large residues (n >= 2^13) are a 2-D split: C-word rows (contiguous) and R-word
-// columns (stride C), with the 4-step twiddle omega_n^(c * kcol) after the column pass.
+// columns (stride C), with the 4-step twiddle omega_n^(c * kcol) between the column and
+// the row transforms.
 //
 // Per iteration (large path), state x and work buffer y (the carry of one column tile
 // reads halo digits of the neighbouring tile, so col_fwd must not write x):
-//   col_fwd : x -> y  carry(+ -2) of the previous pre-carry values, weight, column NTT,
-//                     4-step twiddle
-//   row_sq  : y -> y  row NTT, pointwise square, inverse row NTT
-//   col_inv : y -> x  inverse 4-step tw, inverse column NTT, unweight (includes 1/n)
+//   col_fwd : x -> y  carry(+ -2) of the previous pre-carry values, weight, column NTT
+//   row_sq  : y -> y  4-step twiddle, row NTT, pointwise square, inverse row NTT,
+//                     inverse 4-step twiddle
+//   col_inv : y -> x  inverse column NTT, unweight (includes 1/n)
 // These play the roles of the reference's r2b + m2 + w_fftc_tw (kernels/r2b.cu:25-50,
 // kernels/m2.cu:23-41, kernels/w_fftc_tw__r2_8to4k.cu:20-46), fftr_sqr_ifftr
 // (kernels/fftr_sqr_ifftr__r2_8to4k.cu:20-56) and tw_ifftc_uw
