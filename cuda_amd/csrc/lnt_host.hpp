@@ -94,11 +94,11 @@ inline uint32_t carry_levels(uint32_t p, uint32_t n) {
 // Row/column split of a large residue: log2 C and log2 R.
 inline bool split_for(uint32_t log_n, uint32_t &lc, uint32_t &lr) {
     if (log_n < 13 || log_n > 24) return false;
-    // rows of 4096 words (3-phase row kernel) from n = 2^18 on: measured 1.5-2x faster than
-    // the square-ish splits with 2-phase rows of 1024 or 2048 words
-    lc = log_n >= 18 ? 12 : (log_n + 1) / 2;
+    // rows of 4096 words (3-phase row kernel) from n = 2^16 on: measured 1.2-2x faster than
+    // the square-ish splits with 2-phase rows of 256 .. 2048 words
+    lc = log_n >= 16 ? 12 : (log_n + 1) / 2;
     lr = log_n - lc;
-    return lr <= 12 && lr >= 6;
+    return lr <= 12 && lr >= 4;
 }
 
 // Choices the tuner may vary (0 = default).
@@ -119,8 +119,8 @@ inline bool col3_tile_ok(uint32_t R, uint32_t K) {
 
 // Column tile widths that fit the column kernels' LDS (R * K words of 8 bytes <= 128 KiB).
 inline bool col_tile_ok(uint32_t R, uint32_t K) {
-    static const uint32_t shapes[][2] = {{64, 16}, {128, 16}, {256, 16}, {512, 16}, {1024, 16}, {1024, 8},
-                                         {2048, 8}, {2048, 4}, {4096, 4}};
+    static const uint32_t shapes[][2] = {{16, 16},  {32, 16},  {64, 16}, {128, 16}, {256, 16}, {512, 16},
+                                         {1024, 16}, {1024, 8}, {2048, 8}, {2048, 4}, {4096, 4}};
     for (auto &s : shapes)
         if (s[0] == R && s[1] == K) return true;
     return false;
@@ -152,7 +152,7 @@ inline std::string make_plan(uint32_t p, uint64_t n, Plan &pl, PlanOpts opts = P
     if (force_lc) {
         lc = force_lc;
         lr = pl.log_n - lc;
-        if (lc < 7 || lc > 12 || lr < 6 || lr > 12) return "unsupported row/column split";
+        if (lc < 7 || lc > 12 || lr < 4 || lr > 12) return "unsupported row/column split";
     }
     pl.C = 1u << lc;
     pl.R = 1u << lr;

@@ -19,7 +19,7 @@ static bool go_col_inv(const lnt::Plan &pl, const LntParams &P, hipStream_t s) {
 }
 
 #define COL_SHAPES(X) \
-    X(8, 8, 16) X(16, 8, 16) X(16, 16, 16) X(32, 16, 16) X(32, 32, 16) X(32, 32, 8) X(64, 32, 8) X(64, 32, 4) X(64, 64, 4)
+    X(4, 4, 16) X(8, 4, 16) X(8, 8, 16) X(16, 8, 16) X(16, 16, 16) X(32, 16, 16) X(32, 32, 16) X(32, 32, 8) X(64, 32, 8) X(64, 32, 4) X(64, 64, 4)
 
 // 3-phase column shapes (BA, BB, BC, K)
 #define COL3_SHAPES(X) \

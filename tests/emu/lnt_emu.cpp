@@ -274,8 +274,8 @@ static bool run_large_once(Emu *e, bool sub2, const LntParams &P, int mode = LNT
         ok = true;                                                               \
     }
     if (!pl.col3) {
-        CF(8, 8, 16) CF(16, 8, 16) CF(16, 16, 16) CF(32, 16, 16) CF(32, 32, 16) CF(32, 32, 8) CF(64, 32, 8)
-        CF(64, 32, 4) CF(64, 64, 4)
+        CF(4, 4, 16) CF(8, 4, 16) CF(8, 8, 16) CF(16, 8, 16) CF(16, 16, 16) CF(32, 16, 16) CF(32, 32, 16)
+        CF(32, 32, 8) CF(64, 32, 8) CF(64, 32, 4) CF(64, 64, 4)
     }
 #undef CF
 #define C3F(ba, bb, bc, k)                                                                    \
@@ -308,8 +308,8 @@ static bool run_large_once(Emu *e, bool sub2, const LntParams &P, int mode = LNT
         ok = true;                                                              \
     }
     if (!pl.col3) {
-        CI(8, 8, 16) CI(16, 8, 16) CI(16, 16, 16) CI(32, 16, 16) CI(32, 32, 16) CI(32, 32, 8) CI(64, 32, 8)
-        CI(64, 32, 4) CI(64, 64, 4)
+        CI(4, 4, 16) CI(8, 4, 16) CI(8, 8, 16) CI(16, 8, 16) CI(16, 16, 16) CI(32, 16, 16) CI(32, 32, 16)
+        CI(32, 32, 8) CI(64, 32, 8) CI(64, 32, 4) CI(64, 64, 4)
     }
 #undef CI
 #define C3I(ba, bb, bc, k)                                                    \

@@ -44,6 +44,7 @@ CASES = [  # (p, n, force_lc): small path 8..4096, large path 2^13..2^17 with se
     (19937, 1024, 0), (44497, 2048, 0), (86243, 4096, 0),
     (86243, 8192, 0), (200003, 16384, 0), (400009, 32768, 0), (400009, 32768, 9),
     (1000003, 65536, 0), (1000003, 65536, 10), (2000003, 131072, 0),
+    (1000003, 65536, 12), (2000003, 131072, 12),  # 4096-word rows over 16 / 32-row columns
 ]
 
 
