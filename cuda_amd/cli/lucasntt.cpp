@@ -205,6 +205,7 @@ class LucasPRP {
     void Tune() {
         if (cfg_.performanceLoops <= 0) return;
         check(lnt_tune(ctx_, (int)cfg_.performanceLoops), "lnt_tune");
+        SaveTuning();
         char buf[1024];
         lnt_describe(ctx_, buf, sizeof buf);
         names_.clear();
@@ -226,6 +227,28 @@ class LucasPRP {
                     names_[0] + " = " + fmt_g(ms);
         SAY("\nSummary of the performances of the different processes on %ld loops:", cfg_.performanceLoops);
         SAY("%s = %g", names_[0].c_str(), ms);
+    }
+
+    // The tuner's choices persist in LucasNTT.tune (one line per device and length), so a
+    // later run with PerformanceLoops = 0 starts on the tuned plan.
+    static constexpr const char *kTuneFile = "LucasNTT.tune";
+    static void LoadTuning() {
+        FILE *f = std::fopen(kTuneFile, "r");
+        if (!f) return;
+        std::string text;
+        char buf[512];
+        while (std::fgets(buf, sizeof buf, f)) text += buf;
+        std::fclose(f);
+        if (lnt_tune_import(text.c_str()) != LNT_OK)
+            VERBOSE("Ignoring %s: %s", kTuneFile, lnt_last_error(nullptr));
+    }
+    static void SaveTuning() {
+        std::vector<char> buf(1 << 16);
+        if (lnt_tune_export(buf.data(), buf.size()) != LNT_OK) return;
+        FILE *f = std::fopen(kTuneFile, "w");
+        if (!f) return;
+        std::fputs(buf.data(), f);
+        std::fclose(f);
     }
 
     void Run() {
@@ -474,6 +497,7 @@ int main(int argc, char **argv) {  // main.cpp:22-39
             return 0;
         }
         SAY("");
+        LucasPRP::LoadTuning();
         LucasPRP test(cfg);
         test.Initialize();
         test.Tune();

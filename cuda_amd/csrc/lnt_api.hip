@@ -501,6 +501,48 @@ const char *lnt_last_error(const lnt_ctx *c) {
 
 uint32_t lnt_secured_log2_length(uint32_t p) { return secured_log2_length(p); }
 
+// Tuned plans as text, one line per (device, length): "device n lc K row3 col3".
+int lnt_tune_export(char *buf, size_t len) {
+    if (!buf || !len) return fail(nullptr, LNT_INVALID_ARGUMENT, "bad argument");
+    std::string out;
+    {
+        std::lock_guard<std::mutex> lk(g_tuned_mu);
+        for (const auto &kv : g_tuned) {
+            char line[96];
+            std::snprintf(line, sizeof line, "%d %u %u %u %d %d\n", kv.first.first, kv.first.second, kv.second.lc,
+                          kv.second.K, kv.second.row3, kv.second.col3);
+            out += line;
+        }
+    }
+    std::snprintf(buf, len, "%s", out.c_str());
+    return out.size() < len ? LNT_OK : fail(nullptr, LNT_INVALID_ARGUMENT, "lnt_tune_export: buffer too small");
+}
+
+int lnt_tune_import(const char *text) {
+    if (!text) return fail(nullptr, LNT_INVALID_ARGUMENT, "bad argument");
+    std::vector<std::pair<std::pair<int, uint32_t>, PlanOpts>> rows;
+    const char *s = text;
+    while (*s) {
+        const char *e = std::strchr(s, '\n');
+        std::string line(s, e ? (size_t)(e - s) : std::strlen(s));
+        s = e ? e + 1 : s + line.size();
+        if (line.find_first_not_of(" \t\r") == std::string::npos || line[0] == '#') continue;
+        int dev = 0, r3 = -1, c3 = -1;
+        unsigned n = 0, lc = 0, K = 0;
+        if (std::sscanf(line.c_str(), "%d %u %u %u %d %d", &dev, &n, &lc, &K, &r3, &c3) != 6 || !n || (n & (n - 1)))
+            return fail(nullptr, LNT_INVALID_ARGUMENT, "lnt_tune_import: bad line '" + line + "'");
+        PlanOpts o;
+        o.lc = lc;
+        o.K = K;
+        o.row3 = r3;
+        o.col3 = c3;
+        rows.push_back({{dev, n}, o});
+    }
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    for (auto &r : rows) g_tuned[r.first] = r.second;  // a stale entry falls back to the default plan
+    return LNT_OK;
+}
+
 }  // extern "C"
 
 // Field-op self-test: the device forms of gl64.hpp (carry chains) against the host forms.

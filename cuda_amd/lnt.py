@@ -35,7 +35,7 @@ EXPORTS = (
     "lnt_batch_add", "lnt_batch_remove", "lnt_batch_size", "lnt_batch_iterate", "lnt_batch_sync",
     "lnt_batch_destroy", "lnt_batch_last_error", "lnt_server_create", "lnt_server_submit",
     "lnt_server_poll", "lnt_server_stats", "lnt_server_now", "lnt_server_destroy",
-    "lnt_server_last_error", "lnt_selftest",
+    "lnt_server_last_error", "lnt_selftest", "lnt_tune_export", "lnt_tune_import",
 )
 
 
@@ -111,6 +111,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lnt_batch_last_error.argtypes = [vp]
     lib.lnt_batch_last_error.restype = ctypes.c_char_p
     lib.lnt_selftest.argtypes = [ctypes.c_int, u32, ctypes.c_uint64, u64p]
+    lib.lnt_tune_export.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    lib.lnt_tune_import.argtypes = [ctypes.c_char_p]
     lib.lnt_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
     lib.lnt_server_submit.argtypes = [vp, u32, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64]
     lib.lnt_server_poll.argtypes = [vp, ctypes.POINTER(Result), ctypes.c_int, ctypes.c_int]
@@ -123,6 +125,24 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lnt_server_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
+
+
+def tune_export() -> str:
+    """The tuner's choices so far, one line per device and length."""
+    lib = load()
+    buf = ctypes.create_string_buffer(1 << 16)
+    rc = lib.lnt_tune_export(buf, len(buf))
+    if rc != LNT_OK:
+        raise LntError(rc, lib.lnt_last_error(None).decode())
+    return buf.value.decode()
+
+
+def tune_import(text: str) -> None:
+    """Merge saved tuner choices (the format of tune_export) for later contexts."""
+    lib = load()
+    rc = lib.lnt_tune_import(text.encode())
+    if rc != LNT_OK:
+        raise LntError(rc, lib.lnt_last_error(None).decode())
 
 
 def field_selftest(count: int = 1 << 16, seed: int = 1, device: int = 0) -> int:

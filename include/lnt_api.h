@@ -81,6 +81,13 @@ const char *lnt_last_error(const lnt_ctx *ctx);
  * (NttLengths::GetNttSecuredLength, ntt_lengths.cu:88-98), in exact integer arithmetic. */
 uint32_t lnt_secured_log2_length(uint32_t p);
 
+/* The tuner's choices as text, one line per device and length: "device n lc K row3 col3"
+ * (new: the reference re-tunes at every start, process.cu:45-127).  Export fills buf
+ * (LNT_INVALID_ARGUMENT if it is too small); import merges lines into the choices used
+ * by later lnt_create calls.  Both are host-only. */
+int lnt_tune_export(char *buf, size_t len);
+int lnt_tune_import(const char *text);
+
 /* Self-test of the device field arithmetic (new): add, sub, mul, square and the shift
  * multiplies computed on the GPU for `count` operand pairs (edge values, then splitmix64
  * draws from `seed`) and compared with the host forms; *mismatches = differing results. */

@@ -152,3 +152,21 @@ def test_interrupt_and_resume_equals_uninterrupted(tmp_path):
     assert "Previous residue successfully loaded!" in r.stdout, r.stdout
     assert "M(86243) is prime!" in r.stdout
     assert not (tmp_path / "LucasNTT_results.dat").exists()  # LoadResidue removes the dump
+
+
+@pytest.mark.gpu
+def test_tuning_persists_in_the_tune_file(tmp_path):
+    """A run that tunes writes LucasNTT.tune; a later run without tuning reads it and ends
+    on the same residue."""
+    (tmp_path / "LucasNTT.ini").write_text("PerformanceLoops = 10\n")
+    r = run([200003, 300], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = (tmp_path / "LucasNTT.tune").read_text().splitlines()
+    assert any(l.split()[:2] == ["0", "8192"] for l in lines), lines
+    first = (tmp_path / "LucasNTT_results.txt").read_text()
+    (tmp_path / "LucasNTT_results.txt").unlink()
+    (tmp_path / "LucasNTT.ini").write_text("PerformanceLoops = 0\n")
+    r = run([200003, 300], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    res = [l for l in first.splitlines() if l.startswith("Res64")]
+    assert res and res[0] in (tmp_path / "LucasNTT_results.txt").read_text()

@@ -45,3 +45,14 @@ def test_create_without_gpu_fails_loudly():
         pytest.skip("a GPU is present")
     with pytest.raises(cuda_amd.LntError):
         cuda_amd.LucasPRP(86243)
+
+
+def test_tuner_choices_round_trip_as_text():
+    """lnt_tune_import / lnt_tune_export (host-only): lines in, the same lines out; a
+    malformed line is rejected."""
+    from cuda_amd.lnt import LntError, tune_export, tune_import
+    tune_import("# saved choices\n0 4194304 12 16 -1 -1\n0 65536 12 16 -1 -1\n")  # the default plans
+    out = tune_export().splitlines()
+    assert "0 4194304 12 16 -1 -1" in out and "0 65536 12 16 -1 -1" in out
+    with pytest.raises(LntError):
+        tune_import("0 1000 12 16 1 1\n")  # not a power of two
