@@ -147,7 +147,9 @@ static int build_tables(lnt_ctx *c) {
     hipLaunchKernelGGL(lnt_k_weights, dim3((n + 255) / 256), dim3(256), 0, c->stream, Bw, gl_inv(Bw), ninv, c->p,
                        n, c->w, c->uw);
     u64 omr = gl_pow(554, (GL_P - 1) / Lr);
-    const u32 rB1 = pl.row3 ? pl.row3 : pl.row.B1, rB2 = pl.row3 ? pl.row3 * pl.row3 : pl.row.B2;
+    // row line twiddles: first phase B1 x B2 (3-phase rows: row3 x row3^2, row3m x rBB rBC)
+    const u32 rB1 = pl.row3m ? pl.row3m : pl.row3 ? pl.row3 : pl.row.B1;
+    const u32 rB2 = pl.row3m ? pl.rBB * pl.rBC : pl.row3 ? pl.row3 * pl.row3 : pl.row.B2;
     hipLaunchKernelGGL(lnt_k_line_tw, dim3((Lr + 255) / 256), dim3(256), 0, c->stream, omr, rB1, rB2, Lr, c->twr);
     hipLaunchKernelGGL(lnt_k_line_tw, dim3((Lr + 255) / 256), dim3(256), 0, c->stream, gl_inv(omr), rB1, rB2, Lr,
                        c->itwr);
@@ -183,20 +185,21 @@ static int build_tables(lnt_ctx *c) {
                                pl.col.B1, pl.col.B2, n, c->itw4);
         }
     }
-    if (pl.row3) {
-        const u32 L2 = pl.row3 * pl.row3;
+    // second-phase row twiddles: a line of L2 = B2 words split (row3 x row3 or rBB x rBC)
+    const u32 L2 = pl.row3m ? pl.rBB * pl.rBC : pl.row3 ? pl.row3 * pl.row3 : 0;
+    if (L2) {
+        const u32 s1 = pl.row3m ? pl.rBB : pl.row3, s2 = pl.row3m ? pl.rBC : pl.row3;
         HIPCHK(c, hipMalloc(&c->twr2, (size_t)L2 * 8), "hipMalloc (row twiddles)");
         HIPCHK(c, hipMalloc(&c->itwr2, (size_t)L2 * 8), "hipMalloc (row twiddles)");
         u64 om2 = gl_pow(554, (GL_P - 1) / L2);
-        hipLaunchKernelGGL(lnt_k_line_tw, dim3((L2 + 255) / 256), dim3(256), 0, c->stream, om2, pl.row3, pl.row3, L2,
-                           c->twr2);
-        hipLaunchKernelGGL(lnt_k_line_tw, dim3((L2 + 255) / 256), dim3(256), 0, c->stream, gl_inv(om2), pl.row3,
-                           pl.row3, L2, c->itwr2);
+        hipLaunchKernelGGL(lnt_k_line_tw, dim3((L2 + 255) / 256), dim3(256), 0, c->stream, om2, s1, s2, L2, c->twr2);
+        hipLaunchKernelGGL(lnt_k_line_tw, dim3((L2 + 255) / 256), dim3(256), 0, c->stream, gl_inv(om2), s1, s2, L2,
+                           c->itwr2);
     }
     HIPCHK(c, hipGetLastError(), "table kernels launch");
     LntParams &P = c->P;
-    P.twr2 = view<const u64>(c->twr2, pl.row3 ? pl.row3 * pl.row3 : 0);
-    P.itwr2 = view<const u64>(c->itwr2, pl.row3 ? pl.row3 * pl.row3 : 0);
+    P.twr2 = view<const u64>(c->twr2, L2);
+    P.itwr2 = view<const u64>(c->itwr2, L2);
     P.x = view<u64>(c->x, n);
     P.y = view<u64>(c->y, n);
     P.w = view<const u64>(c->w, n);
@@ -618,7 +621,8 @@ static std::string describe(const Plan &pl) {
         std::snprintf(b, sizeof b, "LNT_1WG;lnt_k_small<%u,%u>", pl.row.B1, pl.row.B2);
     } else {
         char row[64];
-        if (pl.row3) std::snprintf(row, sizeof row, "lnt_k_row3<%u,%u>", pl.row3, pl.G);
+        if (pl.row3m) std::snprintf(row, sizeof row, "lnt_k_row3m<%u,%u,%u>", pl.row3m, pl.rBB, pl.rBC);
+        else if (pl.row3) std::snprintf(row, sizeof row, "lnt_k_row3<%u,%u>", pl.row3, pl.G);
         else std::snprintf(row, sizeof row, "lnt_k_row<%u,%u>", pl.row.B1, pl.row.B2);
         if (pl.col3)
             std::snprintf(b, sizeof b, "LNT_2PASS_%ux%u;lnt_k_col3_fwd<%u,%u,%u,%u>;%s;lnt_k_col3_inv<%u,%u,%u,%u>", pl.R,
@@ -710,7 +714,7 @@ int lnt_tune(lnt_ctx *c, int loops) {
     };
     // candidates: every row/column split, column tile and row variant this build has
     std::vector<PlanOpts> opts;
-    for (uint32_t lc = 7; lc <= 12; lc++)
+    for (uint32_t lc = 7; lc <= 14; lc++)
         for (uint32_t K : {16u, 8u, 4u})
             for (int r3 : {0, 1})
                 for (int c3 : {0, 1}) {

@@ -57,6 +57,8 @@ struct Plan {
     uint32_t K = 0;         // columns per column tile
     uint32_t G = 0;         // rows per row workgroup
     uint32_t row3 = 0;      // 3-phase rows: C = row3^3 (0: 2-phase rows)
+    uint32_t row3m = 0;     // mixed-radix 3-phase rows: C = row3m * rBB * rBC (0: none)
+    uint32_t rBB = 0, rBC = 0;
     uint32_t col3 = 0;      // 3-phase columns: first radix BA (0: 2-phase columns)
     uint32_t cBB = 0, cBC = 0;  // 3-phase columns: second and third radix
     uint32_t M = 0;         // carry halo (compile-time), >= levels
@@ -93,10 +95,11 @@ inline uint32_t carry_levels(uint32_t p, uint32_t n) {
 
 // Row/column split of a large residue: log2 C and log2 R.
 inline bool split_for(uint32_t log_n, uint32_t &lc, uint32_t &lr) {
-    if (log_n < 13 || log_n > 24) return false;
+    if (log_n < 13 || log_n > 26) return false;
     // rows of 4096 words (3-phase row kernel) from n = 2^16 on: measured 1.2-2x faster than
-    // the square-ish splits with 2-phase rows of 256 .. 2048 words
-    lc = log_n >= 16 ? 12 : (log_n + 1) / 2;
+    // the square-ish splits with 2-phase rows of 256 .. 2048 words; above 2^24 the columns
+    // stay at 4096 words and the rows grow to 8192 / 16384 (mixed-radix 3-phase rows)
+    lc = log_n >= 25 ? log_n - 12 : log_n >= 16 ? 12 : (log_n + 1) / 2;
     lr = log_n - lc;
     return lr <= 12 && lr >= 4;
 }
@@ -105,7 +108,8 @@ inline bool split_for(uint32_t log_n, uint32_t &lc, uint32_t &lr) {
 struct PlanOpts {
     uint32_t lc = 0;    // log2 of the row length
     uint32_t K = 0;     // column tile width
-    int row3 = -1;      // 1: 3-phase rows when C is a cube, 0: 2-phase, -1: default
+    int row3 = -1;      // 1: 3-phase rows when C is 512 or 4096, 0: 2-phase, -1: default
+                        // (rows of 8192 and 16384 words are always 3-phase)
     int col3 = -1;      // 1: 3-phase columns (R = 512 .. 4096), 0: 2-phase, -1: default
 };
 
@@ -132,7 +136,7 @@ inline std::string make_plan(uint32_t p, uint64_t n, Plan &pl, PlanOpts opts = P
     if (n < 8 || (n & (n - 1))) return "NTT length must be a power of two >= 8";
     pl.n = (uint32_t)n;
     pl.log_n = ilog2(n);
-    if (pl.log_n > 24) return "NTT lengths above 2^24 are not supported by this engine";
+    if (pl.log_n > 26) return "NTT lengths above 2^26 are not supported by this engine";
     if ((uint64_t)p < 2 * n) return "exponent too small for this NTT length (needs p >= 2n)";
     if ((uint64_t)p / n > 31) return "NTT length too small for this exponent (digits wider than 31 bits)";
     pl.levels = carry_levels(p, (uint32_t)n);
@@ -152,7 +156,7 @@ inline std::string make_plan(uint32_t p, uint64_t n, Plan &pl, PlanOpts opts = P
     if (force_lc) {
         lc = force_lc;
         lr = pl.log_n - lc;
-        if (lc < 7 || lc > 12 || lr < 4 || lr > 12) return "unsupported row/column split";
+        if (lc < 7 || lc > 14 || lr < 4 || lr > 12) return "unsupported row/column split";
     }
     pl.C = 1u << lc;
     pl.R = 1u << lr;
@@ -177,6 +181,15 @@ inline std::string make_plan(uint32_t p, uint64_t n, Plan &pl, PlanOpts opts = P
     }
     pl.G = 64 / pl.row.B2;
     // 3-phase rows by default at C = 4096 (measured faster than the 2-phase rows there)
+    if (pl.C > 4096) {
+        // mixed-radix 3-phase rows, one row per workgroup of rBB * rBC threads
+        if (opts.row3 == 0) return "rows above 4096 words need the 3-phase row kernel";
+        pl.row3m = 32;
+        pl.rBB = pl.C == 8192 ? 16 : 32;
+        pl.rBC = 16;
+        pl.G = 1;
+        return "";
+    }
     const int row3 = opts.row3 >= 0 ? opts.row3 : (pl.C == 4096 ? 1 : 0);
     if (row3 == 1) {
         if (pl.C != 4096 && pl.C != 512) return "3-phase rows need C = 512 or 4096";

@@ -1,4 +1,4 @@
-// Instantiations of the row kernel (row lengths 128 .. 4096).
+// Instantiations of the row kernels (row lengths 128 .. 16384).
 #include "lnt_launch.hpp"
 
 template <int B1, int B2>
@@ -26,7 +26,22 @@ static bool go_row3(const lnt::Plan &pl, const LntParams &P, hipStream_t s, int 
     return true;
 }
 
+template <int BA, int BB, int BC>
+static bool go_row3m(const lnt::Plan &pl, const LntParams &P, hipStream_t s, int mode) {
+    dim3 grid(pl.R), block(BB * BC);
+    if (mode == LNT_FORWARD)
+        hipLaunchKernelGGL((lnt_k_row3m<BA, BB, BC, LNT_FORWARD>), grid, block, 0, s, P);
+    else if (mode == LNT_MULTIPLY)
+        hipLaunchKernelGGL((lnt_k_row3m<BA, BB, BC, LNT_MULTIPLY>), grid, block, 0, s, P);
+    else
+        hipLaunchKernelGGL((lnt_k_row3m<BA, BB, BC, LNT_SQUARE>), grid, block, 0, s, P);
+    return true;
+}
+
 bool lnt_launch_row(const lnt::Plan &pl, const LntParams &P, hipStream_t s, int mode) {
+    if (pl.row3m == 32 && pl.rBB == 16 && pl.rBC == 16) return go_row3m<32, 16, 16>(pl, P, s, mode);
+    if (pl.row3m == 32 && pl.rBB == 32 && pl.rBC == 16) return go_row3m<32, 32, 16>(pl, P, s, mode);
+    if (pl.row3m) return false;
     if (pl.row3 == 16) return go_row3<16, 1>(pl, P, s, mode);
     if (pl.row3 == 8) return go_row3<8, 4>(pl, P, s, mode);
 #define RW(b1, b2) if (pl.row.B1 == b1 && pl.row.B2 == b2) return go_row<b1, b2>(pl, P, s, mode);
@@ -36,6 +51,15 @@ bool lnt_launch_row(const lnt::Plan &pl, const LntParams &P, hipStream_t s, int 
 }
 
 bool lnt_launch_batch_row(const lnt::Plan &pl, const LntParams *Ps, u32 count, hipStream_t s) {
+    if (pl.row3m == 32 && pl.rBB == 16 && pl.rBC == 16) {
+        hipLaunchKernelGGL((lnt_kb_row3m<32, 16, 16>), dim3(pl.R * count), dim3(256), 0, s, Ps, pl.R);
+        return true;
+    }
+    if (pl.row3m == 32 && pl.rBB == 32 && pl.rBC == 16) {
+        hipLaunchKernelGGL((lnt_kb_row3m<32, 32, 16>), dim3(pl.R * count), dim3(512), 0, s, Ps, pl.R);
+        return true;
+    }
+    if (pl.row3m) return false;
     if (pl.row3 == 16) {
         const u32 wgs = pl.R;
         hipLaunchKernelGGL((lnt_kb_row3<16, 1>), dim3(wgs * count), dim3(256), 0, s, Ps, wgs);

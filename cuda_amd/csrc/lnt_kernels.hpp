@@ -571,6 +571,110 @@ LNT_KFN void lnt_row3_body(int tid, int bid, LView lds, const LntParams &P) {
     for (int j = 0; j < B; j++) P.y[base + t + (u64)T * j] = gl_mul(a[j], iv[MODE == LNT_FORWARD ? 0 : j]);
 }
 
+// ------------------------------------------------------------------ mixed-radix 3-phase rows
+// Rows of C = BA * BB * BC words (8192 = 32*16*16, 16384 = 32*32*16), one row per
+// workgroup of A = BB * BC threads holding BA words each; the phases are those of the
+// 3-phase columns (phase 1 over j of words t + A j, phase 2 items (k1, u), phase 3 items
+// (k1, k2)), with the 4-step twiddles at load and store and the square (or product) in
+// the middle.  LDS word of (k1, a = u + BC v): k1 * RS + v * (BC + 1) + u (padded runs).
+template <int BA, int BB, int BC>
+struct LntRow3m {
+    static constexpr int A = BB * BC, C = BA * A;
+    static constexpr int RS = BB * (BC + 1) + 1;
+    static constexpr u32 LDS = (u32)(BA * RS);
+    static_assert(BA % BB == 0 && BA % BC == 0 && BB >= BC, "3-phase row radices");
+};
+
+template <int BA, int BB, int BC, int MODE = LNT_SQUARE>
+LNT_KFN void lnt_row3m_body(int tid, int bid, LView lds, const LntParams &P) {
+    using S = LntRow3m<BA, BB, BC>;
+    constexpr int A = S::A, RS = S::RS;
+    constexpr int QB = BA / BB, QC = BA / BC;
+    const int t = tid;
+    const u64 base = (u64)P.C * (u64)bid;
+    auto pos = [](int k1, int a) { return (u32)(k1 * RS + (a / BC) * (BC + 1) + a % BC); };
+    u64 a[BA];
+#pragma unroll
+    for (int j = 0; j < BA; j++) {
+        const u64 idx = base + t + (u64)A * j;
+        a[j] = gl_mul(P.y[idx], P.tw4[idx]);  // 4-step twiddle of the column pass
+    }
+    // phase 1
+    dft_fwd<BA>(a);
+#pragma unroll
+    for (int rho = 1; rho < BA; rho++) a[rho] = gl_mul(a[rho], P.twr[(u64)rho * A + t]);
+#pragma unroll
+    for (int rho = 0; rho < BA; rho++) lds[pos(brev_c<BA>(rho), t)] = a[rho];
+    LNT_SYNC();
+    // phase 2 (each item rewrites exactly the LDS words it read)
+#pragma unroll
+    for (int q = 0; q < QB; q++) {
+        const int I = q * A + t, k1 = I / BC, u = I % BC;
+        u64 b[BB];
+#pragma unroll
+        for (int v = 0; v < BB; v++) b[v] = lds[pos(k1, u + BC * v)];
+        dft_fwd<BB>(b);
+#pragma unroll
+        for (int rho = 1; rho < BB; rho++) b[rho] = gl_mul(b[rho], P.twr2[(u64)rho * BC + u]);
+#pragma unroll
+        for (int rho = 0; rho < BB; rho++) lds[pos(k1, u + BC * brev_c<BB>(rho))] = b[rho];
+    }
+    LNT_SYNC();
+    // phase 3, the pointwise step, inverse phase 3 (slots t*BA + q*BC + rho)
+#pragma unroll
+    for (int q = 0; q < QC; q++) {
+        const int I = q * A + t, k1 = I / BB, k2 = I % BB;
+        u64 *e = a + q * BC;
+#pragma unroll
+        for (int u = 0; u < BC; u++) e[u] = lds[pos(k1, u + BC * k2)];
+        dft_fwd<BC>(e);
+        if (MODE == LNT_FORWARD) {
+#pragma unroll
+            for (int i = 0; i < BC; i++) P.y[base + (u64)t * BA + q * BC + i] = e[i];
+            continue;
+        }
+        if (MODE == LNT_MULTIPLY) {
+#pragma unroll
+            for (int i = 0; i < BC; i++) e[i] = gl_mul(e[i], P.f[base + (u64)t * BA + q * BC + i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < BC; i++) e[i] = gl_sqr(e[i]);
+        }
+        dft_inv<BC>(e);
+#pragma unroll
+        for (int u = 0; u < BC; u++) lds[pos(k1, u + BC * k2)] = e[u];
+    }
+    if (MODE == LNT_FORWARD) return;
+    LNT_SYNC();
+    // inverse phase 2
+#pragma unroll
+    for (int q = 0; q < QB; q++) {
+        const int I = q * A + t, k1 = I / BC, u = I % BC;
+        u64 b[BB];
+#pragma unroll
+        for (int rho = 0; rho < BB; rho++) {
+            u64 v = lds[pos(k1, u + BC * brev_c<BB>(rho))];
+            b[rho] = rho ? gl_mul(v, P.itwr2[(u64)rho * BC + u]) : v;
+        }
+        dft_inv<BB>(b);
+#pragma unroll
+        for (int v = 0; v < BB; v++) lds[pos(k1, u + BC * v)] = b[v];
+    }
+    LNT_SYNC();
+    // inverse phase 1, inverse 4-step twiddle, store
+#pragma unroll
+    for (int rho = 0; rho < BA; rho++) {
+        u64 v = lds[pos(brev_c<BA>(rho), t)];
+        a[rho] = rho ? gl_mul(v, P.itwr[(u64)rho * A + t]) : v;
+    }
+    dft_inv<BA>(a);
+#pragma unroll
+    for (int j = 0; j < BA; j++) {
+        const u64 idx = base + t + (u64)A * j;
+        P.y[idx] = gl_mul(a[j], P.itw4[idx]);
+    }
+}
+
 // ------------------------------------------------------------------ small residues
 // One workgroup of B2 threads runs `iters` whole iterations on n = B1 * B2 words kept
 // in LDS (natural order, field elements).  LDS words: max(n, B1 * (B2 + PAD)).

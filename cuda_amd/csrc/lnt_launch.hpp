@@ -73,6 +73,13 @@ __global__ __launch_bounds__(G * B * B) void lnt_k_row3(LntParams P) {
     lnt_row3_body<B, G, MODE>(threadIdx.x, blockIdx.x, LView{smem, W}, P);
 }
 
+template <int BA, int BB, int BC, int MODE>
+__global__ __launch_bounds__(BB * BC) void lnt_k_row3m(LntParams P) {
+    constexpr u32 W = LntRow3m<BA, BB, BC>::LDS;
+    __shared__ u64 smem[W];
+    lnt_row3m_body<BA, BB, BC, MODE>(threadIdx.x, blockIdx.x, LView{smem, W}, P);
+}
+
 // ---- batched launches: many tests of one shape in one grid.  Workgroup b serves test
 // b / wgs (its parameters Ps[b / wgs], uniform per workgroup) as that test's workgroup
 // b % wgs.  The "-2" flag is per test (Ps[].sub2).  Each kernel takes a private copy of
@@ -145,6 +152,14 @@ __global__ __launch_bounds__(G * B * B) void lnt_kb_row3(const LntParams *__rest
     __shared__ u64 smem[W];
     const LntParams P = Ps[blockIdx.x / wgs];
     lnt_row3_body<B, G>(threadIdx.x, blockIdx.x % wgs, LView{smem, W}, P);
+}
+
+template <int BA, int BB, int BC>
+__global__ __launch_bounds__(BB * BC) void lnt_kb_row3m(const LntParams *__restrict__ Ps, u32 wgs) {
+    constexpr u32 W = LntRow3m<BA, BB, BC>::LDS;
+    __shared__ u64 smem[W];
+    const LntParams P = Ps[blockIdx.x / wgs];
+    lnt_row3m_body<BA, BB, BC>(threadIdx.x, blockIdx.x % wgs, LView{smem, W}, P);
 }
 
 // Batched launchers: `count` tests of plan pl, parameters in device memory.

@@ -236,7 +236,7 @@ int lnt_server_create(const lnt_server_config *cfg, lnt_server **out) {
 
 int lnt_server_submit(lnt_server *s, uint32_t p, int prp, uint64_t iterations, uint64_t tag) {
     if (!s) return LNT_INVALID_ARGUMENT;
-    if (lnt_secured_log2_length(p) > 24 || p < 67) {
+    if (lnt_secured_log2_length(p) > 26 || p < 67) {
         std::lock_guard<std::mutex> g(s->mu);
         s->err = "exponent outside the supported range";
         return LNT_UNSUPPORTED;

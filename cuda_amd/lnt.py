@@ -246,9 +246,14 @@ class LucasPRP:
         return buf
 
     def set_digits(self, digits: Sequence[int]) -> None:
+        """Digits as a sequence of ints, or (no copy) a contiguous numpy uint64 array."""
         if len(digits) != self.n:
             raise ValueError(f"expected {self.n} digits, got {len(digits)}")
-        buf = (ctypes.c_uint64 * self.n)(*digits)
+        if getattr(digits, "dtype", None) is not None and str(digits.dtype) == "uint64" \
+                and digits.flags["C_CONTIGUOUS"]:
+            buf = digits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        else:
+            buf = (ctypes.c_uint64 * self.n)(*digits)
         self._check(self._lib.lnt_set_digits(self._h, buf, self.n))
 
     def res64(self) -> int:

@@ -36,7 +36,7 @@ typedef enum {
 typedef enum { LNT_MODE_LL = 0, LNT_MODE_PRP = 1 } lnt_mode;
 
 /* Create a test context on `device` for M_p = 2^p - 1 with NTT length
- * n = factors[0] * ... * factors[nfactors-1] (a power of two, 8 <= n <= 2^24).
+ * n = factors[0] * ... * factors[nfactors-1] (a power of two, 8 <= n <= 2^26).
  * Allocates the residue and all tables and seeds the residue with 4 (LL) or 3 (PRP).
  * Replaces LucasPRP::LucasPRP + LucasPRP::Initialize (lucas_prp.cu:24-111) together
  * with the pool/kernel Initialize() table set-up (kernel.cu:91-153, 199-255) and

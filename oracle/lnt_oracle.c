@@ -8,6 +8,9 @@
  *   - field ops           : Python-int arithmetic mod P on random and edge operands
  *   - secured length      : exact integer restatement of the overflow bound, dense sweep
  *   - weights/widths      : B^n = 2, w*uw*n = 1, sum(widths) = p, closed-form ceil(p*i/n)
+ *                           (the reference's double expression equals the integer ceil
+ *                           while p*(i+1) <= 2^53; above that, at n >= 2^25, it rounds and
+ *                           lo_set_exact_widths switches to the integer form, pinned by GMP)
  *   - iterate (LL / PRP)  : GMP mpz residues (oracle/gmp_ll.c fixtures), brute-force
  *                           Python-int runs on tiny p, known Mersenne primes
  *   - carry / minus-two   : brute force vs V mod M_p, including the wrap and borrow cases
@@ -126,6 +129,21 @@ void lo_tables(uint32_t exponent, uint32_t n, uint64_t *weights, uint64_t *unwei
     }
 }
 
+/* Digit widths by integer arithmetic: ceil(p(i+1)/n) - ceil(p i/n) exactly.  Equal to
+ * the reference's double expression above whenever p * n <= 2^53; above that (possible
+ * only at n >= 2^25) the double products round, the expression can return a width one
+ * off at digits with tiny p*i mod n, and the weights (exact integers) then no longer
+ * match the digit layout, so the squaring is not x^2 mod M_p.  Not a reference
+ * restatement: the exact form the weights assume, checked against GMP. */
+void lo_exact_widths(uint32_t exponent, uint32_t n, uint8_t *widths) {
+#pragma omp parallel for schedule(static) if (n >= (1u << 16))
+    for (int64_t tid = 0; tid < (int64_t)n; tid++) {
+        uint64_t a = ((uint64_t)exponent * (uint64_t)(tid + 1) + n - 1) / n;
+        uint64_t b = ((uint64_t)exponent * (uint64_t)tid + n - 1) / n;
+        widths[tid] = (uint8_t)(a - b);
+    }
+}
+
 /* ------------------------------------------------------------------ context */
 
 typedef struct lo_ctx {
@@ -201,6 +219,8 @@ void lo_destroy(lo_ctx *c) {
 }
 
 uint32_t lo_n(const lo_ctx *c) { return c->n; }
+/* Switch a fresh context to exact integer widths (call before setting digits). */
+void lo_set_exact_widths(lo_ctx *c) { lo_exact_widths(c->p, c->n, c->widths); }
 void lo_set_digits(lo_ctx *c, const uint64_t *d) { memcpy(c->x, d, 8ull * c->n); }
 void lo_get_digits(const lo_ctx *c, uint64_t *d) { memcpy(d, c->x, 8ull * c->n); }
 void lo_get_widths(const lo_ctx *c, uint8_t *w) { memcpy(w, c->widths, c->n); }

@@ -69,7 +69,15 @@ extern "C" void *emu_create3(uint32_t p, uint32_t n, int prp, uint32_t lc, uint3
         e->uw[i] = gl_mul(gl_inv(e->w[i]), ninv);
     }
     widths(p, n, e->wid);
-    if (pl.row3) {
+    if (pl.row3m) {
+        Line l1, l2;
+        l1.B1 = pl.row3m;
+        l1.B2 = pl.rBB * pl.rBC;
+        l2.B1 = pl.rBB;
+        l2.B2 = pl.rBC;
+        line_tables(pl.C, l1, e->twr, e->itwr);
+        line_tables(pl.rBB * pl.rBC, l2, e->twr2, e->itwr2);
+    } else if (pl.row3) {
         Line l1, l2;
         l1.B1 = pl.row3;
         l1.B2 = pl.row3 * pl.row3;
@@ -229,6 +237,17 @@ static void row3_sq(Emu *e, const LntParams &P, int mode) {
         });
 }
 
+template <int BA, int BB, int BC>
+static void row3m_sq(Emu *e, const LntParams &P, int mode) {
+    const Plan &pl = e->pl;
+    for (uint32_t b = 0; b < pl.R; b++)
+        run_wg(BB * BC, LntRow3m<BA, BB, BC>::LDS, [&](int tid, LView L) {
+            if (mode == LNT_FORWARD) lnt_row3m_body<BA, BB, BC, LNT_FORWARD>(tid, b, L, P);
+            else if (mode == LNT_MULTIPLY) lnt_row3m_body<BA, BB, BC, LNT_MULTIPLY>(tid, b, L, P);
+            else lnt_row3m_body<BA, BB, BC>(tid, b, L, P);
+        });
+}
+
 template <int B1, int B2, int G>
 static void row_sq(Emu *e, const LntParams &P, int mode) {
     const Plan &pl = e->pl;
@@ -290,10 +309,12 @@ static bool run_large_once(Emu *e, bool sub2, const LntParams &P, int mode = LNT
 #undef C3F
     if (!ok) return false;
     ok = false;
+    if (pl.row3m == 32 && pl.rBB == 16 && pl.rBC == 16) row3m_sq<32, 16, 16>(e, P, mode), ok = true;
+    if (pl.row3m == 32 && pl.rBB == 32 && pl.rBC == 16) row3m_sq<32, 32, 16>(e, P, mode), ok = true;
     if (pl.row3 == 16) row3_sq<16, 1>(e, P, mode), ok = true;
     if (pl.row3 == 8) row3_sq<8, 4>(e, P, mode), ok = true;
 #define RW(b1, b2)                                                        \
-    if (!ok && pl.row.B1 == b1 && pl.row.B2 == b2) {                      \
+    if (!ok && !pl.row3m && pl.row.B1 == b1 && pl.row.B2 == b2) {                      \
         row_sq<b1, b2, 64 / b2>(e, P, mode);                              \
         ok = true;                                                        \
     }
@@ -361,6 +382,18 @@ extern "C" void emu_set_digits(void *h, const uint64_t *d) {
 
 extern "C" uint32_t emu_levels(void *h) { return ((Emu *)h)->pl.levels; }
 extern "C" uint32_t emu_secured(uint32_t p) { return secured_log2_length(p); }
+
+// The plan make_plan picks (no tables): C, R, row3, row3m, rBB, rBC, col3, K; 0 or -1 if rejected.
+extern "C" int emu_plan(uint32_t p, uint32_t n, int lc, int row3, uint32_t *out) {
+    PlanOpts o;
+    o.lc = lc;
+    o.row3 = row3;
+    Plan pl;
+    if (!make_plan(p, n, pl, o).empty()) return -1;
+    const uint32_t v[8] = {pl.C, pl.R, pl.row3, pl.row3m, pl.rBB, pl.rBC, pl.col3, pl.K};
+    for (int i = 0; i < 8; i++) out[i] = v[i];
+    return 0;
+}
 
 // dst <- dst * src (same shape), through the forward / multiply kernel modes.
 extern "C" int emu_mul(void *hd, void *hs) {

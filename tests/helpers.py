@@ -34,6 +34,8 @@ def oracle():
         L.lo_secured_length.restype = u32
         L.lo_candidates.argtypes = [u32, ctypes.c_char_p, ctypes.c_int]
         L.lo_tables.argtypes = [u32, u32, vp, vp, vp]
+        L.lo_exact_widths.argtypes = [u32, u32, vp]
+        L.lo_set_exact_widths.argtypes = [vp]
         L.lo_create.argtypes = [u32, u32, u32, ctypes.c_int]
         L.lo_create.restype = vp
         L.lo_destroy.argtypes = [vp]
@@ -54,11 +56,13 @@ def oracle():
 class OracleTest:
     """The oracle's LucasPRP restatement (n0 = 1: one transform, else the 2-factor chain)."""
 
-    def __init__(self, p, n, prp=False, n0=1):
+    def __init__(self, p, n, prp=False, n0=1, exact_widths=False):
         L = oracle()
         self.L = L
         self.n = n
         self.h = L.lo_create(p, n0, n // n0, int(prp))
+        if exact_widths:  # integer digit widths (differ from the reference's only when p*n > 2^53)
+            L.lo_set_exact_widths(self.h)
 
     def __del__(self):
         try:
@@ -80,6 +84,17 @@ class OracleTest:
     def set_digits(self, d):
         b = (ctypes.c_uint64 * self.n)(*d)
         self.L.lo_set_digits(self.h, b)
+
+    def digits_np(self):
+        import numpy as np
+        b = np.empty(self.n, dtype=np.uint64)
+        self.L.lo_get_digits(self.h, b.ctypes.data_as(ctypes.c_void_p))
+        return b
+
+    def set_digits_np(self, d):
+        import numpy as np
+        d = np.ascontiguousarray(d, dtype=np.uint64)
+        self.L.lo_set_digits(self.h, d.ctypes.data_as(ctypes.c_void_p))
 
     def widths(self):
         b = (ctypes.c_uint8 * self.n)()
@@ -141,6 +156,21 @@ def widths(p, n):
     return [e[i + 1] - e[i] for i in range(n)]
 
 
+def widths_np(p, n):
+    import numpy as np
+    i = np.arange(n + 1, dtype=np.int64)
+    e = -((-p * i) // n)
+    return np.diff(e).astype(np.uint64)
+
+
+def random_digits_np(p, n, seed):
+    """Seeded random natural digits (each below 2^width) as a numpy uint64 array."""
+    import numpy as np
+    w = widths_np(p, n)
+    r = np.random.default_rng(seed).integers(0, 1 << 63, n, dtype=np.uint64)
+    return r & ((np.uint64(1) << w) - np.uint64(1))
+
+
 def digits_of(value, p, n):
     """Natural digits of an integer value (0 <= value < 2^p), in linear time."""
     b = bin(value)[2:].zfill(p)[::-1]  # bit i at index i
@@ -162,3 +192,67 @@ def ll_python(p, k, s=4, prp=False):
         s = s * s if prp else s * s - 2
         s %= M
     return s
+
+
+# --------------------------------------------------------------- GMP (libgmp.so.10)
+# One squaring mod M_p through GMP's documented mpz entry points (no gmp.h in the image;
+# mpz_t is the documented {int alloc; int size; limb *d} struct), for checks at lengths
+# where Python integers are too slow.  Test infrastructure only.
+class _Mpz(ctypes.Structure):
+    _fields_ = [("alloc", ctypes.c_int), ("size", ctypes.c_int), ("d", ctypes.c_void_p)]
+
+
+def _bit_offsets(p, n):
+    import numpy as np
+    w = widths_np(p, n).astype(np.int64)
+    return w, np.concatenate([[0], np.cumsum(w)])
+
+
+def gmp_square_digits(d, p, n, minus_two=True):
+    """Canonical digits (numpy uint64, widths ceil(p(i+1)/n) - ceil(p i/n)) of
+    x^2 - 2 (or x^2) mod 2^p - 1, x the value of the canonical digits d."""
+    import numpy as np
+    G = ctypes.CDLL("libgmp.so.10")
+    vp = ctypes.c_void_p
+    G.__gmpz_import.argtypes = [vp, ctypes.c_size_t, ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
+                                ctypes.c_size_t, vp]
+    G.__gmpz_export.argtypes = [vp, vp, ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_size_t, vp]
+    G.__gmpz_export.restype = vp
+    G.__gmpz_sizeinbase.argtypes = [vp, ctypes.c_int]
+    G.__gmpz_sizeinbase.restype = ctypes.c_size_t
+    G.__gmpz_cmp.argtypes = [vp, vp]
+    w, e = _bit_offsets(p, n)
+    wmax = int(w.max())
+    bits = np.zeros(p, np.uint8)
+    for b in range(wmax):
+        m = w > b
+        bits[e[:-1][m] + b] = ((d[m] >> np.uint64(b)) & np.uint64(1)).astype(np.uint8)
+    by = np.packbits(bits, bitorder="little")
+    x, s, hi, lo, M = _Mpz(), _Mpz(), _Mpz(), _Mpz(), _Mpz()
+    for z in (x, s, hi, lo, M):
+        G.__gmpz_init(ctypes.byref(z))
+    G.__gmpz_import(ctypes.byref(x), len(by), -1, 1, 0, 0, by.ctypes.data_as(vp))
+    G.__gmpz_mul(ctypes.byref(s), ctypes.byref(x), ctypes.byref(x))
+    G.__gmpz_setbit(ctypes.byref(M), ctypes.c_ulong(p))
+    G.__gmpz_sub_ui(ctypes.byref(M), ctypes.byref(M), ctypes.c_ulong(1))
+    if minus_two:
+        G.__gmpz_add(ctypes.byref(s), ctypes.byref(s), ctypes.byref(M))  # stays >= 0
+        G.__gmpz_sub_ui(ctypes.byref(s), ctypes.byref(s), ctypes.c_ulong(2))
+    while G.__gmpz_sizeinbase(ctypes.byref(s), 2) > p:  # fold 2^p = 1
+        G.__gmpz_tdiv_q_2exp(ctypes.byref(hi), ctypes.byref(s), ctypes.c_ulong(p))
+        G.__gmpz_tdiv_r_2exp(ctypes.byref(lo), ctypes.byref(s), ctypes.c_ulong(p))
+        G.__gmpz_add(ctypes.byref(s), ctypes.byref(lo), ctypes.byref(hi))
+    if G.__gmpz_cmp(ctypes.byref(s), ctypes.byref(M)) == 0:  # M_p itself is 0
+        G.__gmpz_sub(ctypes.byref(s), ctypes.byref(s), ctypes.byref(M))
+    buf = np.zeros((p + 7) // 8 + 8, np.uint8)
+    cnt = ctypes.c_size_t()
+    G.__gmpz_export(buf.ctypes.data_as(vp), ctypes.byref(cnt), -1, 1, 0, 0, ctypes.byref(s))
+    for z in (x, s, hi, lo, M):
+        G.__gmpz_clear(ctypes.byref(z))
+    ob = np.unpackbits(buf, bitorder="little")[:p]
+    out = np.zeros(n, np.uint64)
+    for b in range(wmax):
+        m = w > b
+        out[m] |= ob[e[:-1][m] + b].astype(np.uint64) << np.uint64(b)
+    return out
+

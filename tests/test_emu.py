@@ -45,6 +45,7 @@ CASES = [  # (p, n, force_lc): small path 8..4096, large path 2^13..2^17 with se
     (86243, 8192, 0), (200003, 16384, 0), (400009, 32768, 0), (400009, 32768, 9),
     (1000003, 65536, 0), (1000003, 65536, 10), (2000003, 131072, 0),
     (1000003, 65536, 12), (2000003, 131072, 12),  # 4096-word rows over 16 / 32-row columns
+    (2000003, 131072, 13), (4000037, 262144, 14),  # 8192 / 16384-word rows (mixed radix)
 ]
 
 
@@ -118,9 +119,10 @@ def test_emulated_large_path_orbit_matches_oracle():
         assert e.digits() == o.digits(), k
 
 
-@pytest.mark.parametrize("p,n,prp", [(521, 32, True), (9689, 512, False), (86243, 4096, True),
-                                      (86243, 8192, True), (400009, 32768, False)])
-def test_emulated_product_matches_python(p, n, prp):
+@pytest.mark.parametrize("p,n,prp,lc", [(521, 32, True, 0), (9689, 512, False, 0), (86243, 4096, True, 0),
+                                         (86243, 8192, True, 0), (400009, 32768, False, 0),
+                                         (1000003, 131072, True, 13)])
+def test_emulated_product_matches_python(p, n, prp, lc):
     """The forward / multiply kernel modes (used by the Gerbicz check): x <- x * y mod M_p
     from pre-carry states with a pending LL -2, then one more squaring."""
     from helpers import digits_of, value_of
@@ -129,7 +131,7 @@ def test_emulated_product_matches_python(p, n, prp):
     rnd = random.Random(n + p)
     M = (1 << p) - 1
     a, b = rnd.randrange(M), rnd.randrange(M)
-    ea, eb = Emu(p, n, prp), Emu(p, n, prp)
+    ea, eb = Emu(p, n, prp, lc), Emu(p, n, prp, lc)
     ea.set_digits(digits_of(a, p, n))
     eb.set_digits(digits_of(b, p, n))
     ea.iterate(1)
@@ -142,3 +144,20 @@ def test_emulated_product_matches_python(p, n, prp):
     assert value_of(ea.digits(), p, n) == a * b % M
     ea.iterate(1)
     assert value_of(ea.digits(), p, n) == ((a * b % M) ** 2 - c) % M
+
+
+def test_plans_for_the_longest_lengths():
+    """n = 2^25 and 2^26 split into 4096-word columns and 8192 / 16384-word rows (mixed-radix
+    3-phase rows 32 x 16 x 16 and 32 x 32 x 16); longer transforms and 2-phase rows of those
+    lengths are rejected."""
+    L = emu()
+    L.emu_plan.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    out = (ctypes.c_uint32 * 8)()
+    assert L.emu_plan(280000001, 1 << 25, 0, -1, out) == 0
+    assert list(out) == [8192, 4096, 0, 32, 16, 16, 16, 4]
+    assert L.emu_plan(560000017, 1 << 26, 0, -1, out) == 0
+    assert list(out) == [16384, 4096, 0, 32, 32, 16, 16, 4]
+    assert L.emu_plan(140000041, 1 << 24, 13, -1, out) == 0 and out[3] == 32 and out[1] == 2048
+    assert L.emu_plan(560000017, 1 << 26, 0, 0, out) == -1  # no 2-phase rows of 16384 words
+    assert L.emu_plan(1100000017, 1 << 27, 0, -1, out) == -1
+    assert L.emu_secured(560000017) <= 26

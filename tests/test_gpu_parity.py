@@ -75,6 +75,43 @@ def test_large_random_state_matches_oracle(p, n, lc, prp):
     t.close()
 
 
+@pytest.mark.parametrize("p,n", [(600000001, 1 << 25), (1200000001, 1 << 26)])
+def test_longest_lengths_match_oracle(p, n):
+    """n = 2^25 and 2^26 (rows of 8192 and 16384 words, mixed-radix 3-phase row kernel,
+    over 4096-word 3-phase columns) at exponents near the top of each length: two
+    iterations from a random state, digit for digit against the oracle with exact digit
+    widths (the reference's double-precision widths round at these sizes)."""
+    import numpy as np
+    from helpers import random_digits_np
+    say("longest lengths", p, n)
+    d = random_digits_np(p, n, p)
+    t = cuda_amd.LucasPRP(p)
+    assert t.n == n and any(k.startswith("lnt_k_row3m") for k in t.describe())
+    t.set_digits(d)
+    t.iterate(2)
+    got = np.frombuffer(t.get_digits_buffer(), dtype=np.uint64)
+    t.close()
+    o = OracleTest(p, n, False, n0=1024, exact_widths=True)
+    o.set_digits_np(d)
+    o.iterate(2)
+    assert np.array_equal(got, o.digits_np())
+
+
+def test_longest_length_squaring_matches_gmp():
+    """One LL squaring at n = 2^25, p = 600,000,001, against GMP's mpz product."""
+    import numpy as np
+    from helpers import gmp_square_digits, random_digits_np
+    p, n = 600000001, 1 << 25
+    say("GMP squaring", p, n)
+    d = random_digits_np(p, n, 7)
+    t = cuda_amd.LucasPRP(p)
+    t.set_digits(d)
+    t.iterate(1)
+    got = np.frombuffer(t.get_digits_buffer(), dtype=np.uint64)
+    t.close()
+    assert np.array_equal(got, gmp_square_digits(d, p, n))
+
+
 def test_c2_res64_checkpoints():
     """p = 70,000,027, n = 2^22 (the throughput configuration): res64 at every 1,000
     iterations that GMP has been run to."""

@@ -83,6 +83,39 @@ def test_tables_closed_form(p, n):
         assert w[i] * uw[i] % P_MOD * n % P_MOD == 1
 
 
+@pytest.mark.parametrize("p,n", [(318767093, 1 << 24), (83886071, 1 << 22), (280000001, 1 << 25)])
+def test_reference_width_expression_exact_up_to_2p53(p, n):
+    """The reference's double expression ceil(1.0*p*(i+1)/n) - ceil(1.0*p*i/n) equals the
+    integer form while p * n <= 2^53 (every length up to 2^24 at its largest exponent)."""
+    import numpy as np
+    from helpers import widths_np
+    L = oracle()
+    a, b = np.empty(n, np.uint8), np.empty(n, np.uint8)
+    L.lo_tables(p, n, None, None, a.ctypes.data_as(ctypes.c_void_p))
+    L.lo_exact_widths(p, n, b.ctypes.data_as(ctypes.c_void_p))
+    assert np.array_equal(a, b) and np.array_equal(b, widths_np(p, n).astype(np.uint8))
+
+
+@pytest.mark.parametrize("p,n,where", [(600000001, 1 << 25, [23116288, 23116289]),
+                                       (560000017, 1 << 26, [41518320, 41518321]),
+                                       (1200000001, 1 << 26, [17396736, 17396737, 34793473, 34793474,
+                                                              52190210, 52190211])])
+def test_reference_width_expression_rounds_above_2p53(p, n, where):
+    """Above p * n = 2^53 (n >= 2^25) the double products round: the reference's expression
+    moves a digit boundary by one bit where p*i mod n is 1, 2 or 3, while the weights (exact
+    integers) assume the exact boundary.  The exact widths are the integer form."""
+    import numpy as np
+    from helpers import widths_np
+    L = oracle()
+    a, b = np.empty(n, np.uint8), np.empty(n, np.uint8)
+    L.lo_tables(p, n, None, None, a.ctypes.data_as(ctypes.c_void_p))
+    L.lo_exact_widths(p, n, b.ctypes.data_as(ctypes.c_void_p))
+    assert np.array_equal(b, widths_np(p, n).astype(np.uint8))
+    bad = np.nonzero(a != b)[0].tolist()
+    assert bad == where
+    assert all(min((p * i) % n, (p * (i + 1)) % n) <= 3 for i in bad)
+
+
 @pytest.mark.parametrize("case", [c for c in golden_cases() if c["p"] < 30000],
                          ids=lambda c: f"{c['mode']}{c['p']}")
 def test_oracle_matches_gmp_small(case):
