@@ -716,7 +716,7 @@ int lnt_tune(lnt_ctx *c, int loops) {
     std::vector<PlanOpts> opts;
     for (uint32_t lc = 7; lc <= 14; lc++)
         for (uint32_t K : {16u, 8u, 4u})
-            for (int r3 : {0, 1})
+            for (int r3 : {0, 1, 2})
                 for (int c3 : {0, 1}) {
                     PlanOpts o;
                     o.lc = lc;

@@ -109,7 +109,8 @@ struct PlanOpts {
     uint32_t lc = 0;    // log2 of the row length
     uint32_t K = 0;     // column tile width
     int row3 = -1;      // 1: 3-phase rows when C is 512 or 4096, 0: 2-phase, -1: default
-                        // (rows of 8192 and 16384 words are always 3-phase)
+                        // (rows of 8192 and 16384 words are always 3-phase: 1 and 2
+                        // select 32 or 16 words per thread)
     int col3 = -1;      // 1: 3-phase columns (R = 512 .. 4096), 0: 2-phase, -1: default
 };
 
@@ -182,15 +183,21 @@ inline std::string make_plan(uint32_t p, uint64_t n, Plan &pl, PlanOpts opts = P
     pl.G = 64 / pl.row.B2;
     // 3-phase rows by default at C = 4096 (measured faster than the 2-phase rows there)
     if (pl.C > 4096) {
-        // mixed-radix 3-phase rows, one row per workgroup of rBB * rBC threads
+        // mixed-radix 3-phase rows, one row per workgroup of rBB * rBC threads:
+        // row3 = 1: 32 words per thread (8192 = 32*16*16, 16384 = 32*32*16);
+        // row3 = 2: 16 words per thread (8192 = 16*32*16, 16384 = 16*32*32).
+        // Default: 16 words at C = 8192 (row kernel 592 -> 540 us at n = 2^25), 32 words
+        // at C = 16384 (the two within 4% there)
         if (opts.row3 == 0) return "rows above 4096 words need the 3-phase row kernel";
-        pl.row3m = 32;
-        pl.rBB = pl.C == 8192 ? 16 : 32;
-        pl.rBC = 16;
+        const bool w16 = opts.row3 == 2 || (opts.row3 < 0 && pl.C == 8192);
+        pl.row3m = w16 ? 16 : 32;
+        pl.rBB = (pl.C == 8192 && !w16) ? 16 : 32;
+        pl.rBC = (pl.C == 16384 && w16) ? 32 : 16;
         pl.G = 1;
         return "";
     }
     const int row3 = opts.row3 >= 0 ? opts.row3 : (pl.C == 4096 ? 1 : 0);
+    if (row3 == 2) return "16-word mixed-radix rows are for rows above 4096 words";
     if (row3 == 1) {
         if (pl.C != 4096 && pl.C != 512) return "3-phase rows need C = 512 or 4096";
         pl.row3 = pl.C == 4096 ? 16 : 8;

@@ -39,8 +39,10 @@ static bool go_row3m(const lnt::Plan &pl, const LntParams &P, hipStream_t s, int
 }
 
 bool lnt_launch_row(const lnt::Plan &pl, const LntParams &P, hipStream_t s, int mode) {
-    if (pl.row3m == 32 && pl.rBB == 16 && pl.rBC == 16) return go_row3m<32, 16, 16>(pl, P, s, mode);
-    if (pl.row3m == 32 && pl.rBB == 32 && pl.rBC == 16) return go_row3m<32, 32, 16>(pl, P, s, mode);
+#define R3M(ba, bb, bc) \
+    if (pl.row3m == ba && pl.rBB == bb && pl.rBC == bc) return go_row3m<ba, bb, bc>(pl, P, s, mode);
+    R3M(32, 16, 16) R3M(32, 32, 16) R3M(16, 32, 16) R3M(16, 32, 32)
+#undef R3M
     if (pl.row3m) return false;
     if (pl.row3 == 16) return go_row3<16, 1>(pl, P, s, mode);
     if (pl.row3 == 8) return go_row3<8, 4>(pl, P, s, mode);
@@ -51,14 +53,13 @@ bool lnt_launch_row(const lnt::Plan &pl, const LntParams &P, hipStream_t s, int 
 }
 
 bool lnt_launch_batch_row(const lnt::Plan &pl, const LntParams *Ps, u32 count, hipStream_t s) {
-    if (pl.row3m == 32 && pl.rBB == 16 && pl.rBC == 16) {
-        hipLaunchKernelGGL((lnt_kb_row3m<32, 16, 16>), dim3(pl.R * count), dim3(256), 0, s, Ps, pl.R);
-        return true;
+#define R3M(ba, bb, bc)                                                                                   \
+    if (pl.row3m == ba && pl.rBB == bb && pl.rBC == bc) {                                                  \
+        hipLaunchKernelGGL((lnt_kb_row3m<ba, bb, bc>), dim3(pl.R * count), dim3(bb * bc), 0, s, Ps, pl.R); \
+        return true;                                                                                       \
     }
-    if (pl.row3m == 32 && pl.rBB == 32 && pl.rBC == 16) {
-        hipLaunchKernelGGL((lnt_kb_row3m<32, 32, 16>), dim3(pl.R * count), dim3(512), 0, s, Ps, pl.R);
-        return true;
-    }
+    R3M(32, 16, 16) R3M(32, 32, 16) R3M(16, 32, 16) R3M(16, 32, 32)
+#undef R3M
     if (pl.row3m) return false;
     if (pl.row3 == 16) {
         const u32 wgs = pl.R;

@@ -572,24 +572,29 @@ LNT_KFN void lnt_row3_body(int tid, int bid, LView lds, const LntParams &P) {
 }
 
 // ------------------------------------------------------------------ mixed-radix 3-phase rows
-// Rows of C = BA * BB * BC words (8192 = 32*16*16, 16384 = 32*32*16), one row per
-// workgroup of A = BB * BC threads holding BA words each; the phases are those of the
-// 3-phase columns (phase 1 over j of words t + A j, phase 2 items (k1, u), phase 3 items
-// (k1, k2)), with the 4-step twiddles at load and store and the square (or product) in
-// the middle.  LDS word of (k1, a = u + BC v): k1 * RS + v * (BC + 1) + u (padded runs).
+// Rows of C = BA * BB * BC words (8192 = 32*16*16 or 16*32*16, 16384 = 32*32*16 or
+// 16*32*32), one row per workgroup of A = BB * BC threads holding BA words each.  The
+// phases are those of the 3-phase columns (phase 1 over j of words t + A j; phase 2 items
+// (k1, u), BA * BC of them; phase 3 items (k1, k2), BA * BB of them), with the 4-step
+// twiddles at load and store and the square (or product) in the middle.  With BA = 16 a
+// phase may have fewer items than threads (the rest wait at the barrier): 16 words per
+// thread keep the register count low enough for twice the waves of BA = 32.
+// LDS word of (k1, a = u + BC v): k1 * RS + v * (BC + 1) + u (padded runs).
 template <int BA, int BB, int BC>
 struct LntRow3m {
     static constexpr int A = BB * BC, C = BA * A;
+    static constexpr int N2 = BA * BC, N3 = BA * BB;  // phase 2 / phase 3 items
+    static constexpr int Q2 = (N2 + A - 1) / A, Q3 = (N3 + A - 1) / A;  // items per thread
     static constexpr int RS = BB * (BC + 1) + 1;
     static constexpr u32 LDS = (u32)(BA * RS);
-    static_assert(BA % BB == 0 && BA % BC == 0 && BB >= BC, "3-phase row radices");
+    static_assert(N2 % A == 0 || A % N2 == 0, "phase 2 items");
+    static_assert(N3 % A == 0 || A % N3 == 0, "phase 3 items");
 };
 
 template <int BA, int BB, int BC, int MODE = LNT_SQUARE>
 LNT_KFN void lnt_row3m_body(int tid, int bid, LView lds, const LntParams &P) {
     using S = LntRow3m<BA, BB, BC>;
-    constexpr int A = S::A, RS = S::RS;
-    constexpr int QB = BA / BB, QC = BA / BC;
+    constexpr int A = S::A, RS = S::RS, N2 = S::N2, N3 = S::N3, Q2 = S::Q2, Q3 = S::Q3;
     const int t = tid;
     const u64 base = (u64)P.C * (u64)bid;
     auto pos = [](int k1, int a) { return (u32)(k1 * RS + (a / BC) * (BC + 1) + a % BC); };
@@ -608,57 +613,65 @@ LNT_KFN void lnt_row3m_body(int tid, int bid, LView lds, const LntParams &P) {
     LNT_SYNC();
     // phase 2 (each item rewrites exactly the LDS words it read)
 #pragma unroll
-    for (int q = 0; q < QB; q++) {
+    for (int q = 0; q < Q2; q++) {
         const int I = q * A + t, k1 = I / BC, u = I % BC;
-        u64 b[BB];
+        if (N2 >= A || I < N2) {
+            u64 b[BB];
 #pragma unroll
-        for (int v = 0; v < BB; v++) b[v] = lds[pos(k1, u + BC * v)];
-        dft_fwd<BB>(b);
+            for (int v = 0; v < BB; v++) b[v] = lds[pos(k1, u + BC * v)];
+            dft_fwd<BB>(b);
 #pragma unroll
-        for (int rho = 1; rho < BB; rho++) b[rho] = gl_mul(b[rho], P.twr2[(u64)rho * BC + u]);
+            for (int rho = 1; rho < BB; rho++) b[rho] = gl_mul(b[rho], P.twr2[(u64)rho * BC + u]);
 #pragma unroll
-        for (int rho = 0; rho < BB; rho++) lds[pos(k1, u + BC * brev_c<BB>(rho))] = b[rho];
+            for (int rho = 0; rho < BB; rho++) lds[pos(k1, u + BC * brev_c<BB>(rho))] = b[rho];
+        }
     }
     LNT_SYNC();
-    // phase 3, the pointwise step, inverse phase 3 (slots t*BA + q*BC + rho)
+    // phase 3, the pointwise step, inverse phase 3 (item q of thread t: slots
+    // (q * A + t) * BC + rho, in the order of the items)
 #pragma unroll
-    for (int q = 0; q < QC; q++) {
+    for (int q = 0; q < Q3; q++) {
         const int I = q * A + t, k1 = I / BB, k2 = I % BB;
-        u64 *e = a + q * BC;
+        if (N3 >= A || I < N3) {
+            u64 e[BC];
 #pragma unroll
-        for (int u = 0; u < BC; u++) e[u] = lds[pos(k1, u + BC * k2)];
-        dft_fwd<BC>(e);
-        if (MODE == LNT_FORWARD) {
+            for (int u = 0; u < BC; u++) e[u] = lds[pos(k1, u + BC * k2)];
+            dft_fwd<BC>(e);
+            const u64 so = base + (u64)I * BC;
+            if (MODE == LNT_FORWARD) {
 #pragma unroll
-            for (int i = 0; i < BC; i++) P.y[base + (u64)t * BA + q * BC + i] = e[i];
-            continue;
+                for (int i = 0; i < BC; i++) P.y[so + i] = e[i];
+                continue;
+            }
+            if (MODE == LNT_MULTIPLY) {
+#pragma unroll
+                for (int i = 0; i < BC; i++) e[i] = gl_mul(e[i], P.f[so + i]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < BC; i++) e[i] = gl_sqr(e[i]);
+            }
+            dft_inv<BC>(e);
+#pragma unroll
+            for (int u = 0; u < BC; u++) lds[pos(k1, u + BC * k2)] = e[u];
         }
-        if (MODE == LNT_MULTIPLY) {
-#pragma unroll
-            for (int i = 0; i < BC; i++) e[i] = gl_mul(e[i], P.f[base + (u64)t * BA + q * BC + i]);
-        } else {
-#pragma unroll
-            for (int i = 0; i < BC; i++) e[i] = gl_sqr(e[i]);
-        }
-        dft_inv<BC>(e);
-#pragma unroll
-        for (int u = 0; u < BC; u++) lds[pos(k1, u + BC * k2)] = e[u];
     }
     if (MODE == LNT_FORWARD) return;
     LNT_SYNC();
     // inverse phase 2
 #pragma unroll
-    for (int q = 0; q < QB; q++) {
+    for (int q = 0; q < Q2; q++) {
         const int I = q * A + t, k1 = I / BC, u = I % BC;
-        u64 b[BB];
+        if (N2 >= A || I < N2) {
+            u64 b[BB];
 #pragma unroll
-        for (int rho = 0; rho < BB; rho++) {
-            u64 v = lds[pos(k1, u + BC * brev_c<BB>(rho))];
-            b[rho] = rho ? gl_mul(v, P.itwr2[(u64)rho * BC + u]) : v;
+            for (int rho = 0; rho < BB; rho++) {
+                u64 v = lds[pos(k1, u + BC * brev_c<BB>(rho))];
+                b[rho] = rho ? gl_mul(v, P.itwr2[(u64)rho * BC + u]) : v;
+            }
+            dft_inv<BB>(b);
+#pragma unroll
+            for (int v = 0; v < BB; v++) lds[pos(k1, u + BC * v)] = b[v];
         }
-        dft_inv<BB>(b);
-#pragma unroll
-        for (int v = 0; v < BB; v++) lds[pos(k1, u + BC * v)] = b[v];
     }
     LNT_SYNC();
     // inverse phase 1, inverse 4-step twiddle, store

@@ -73,8 +73,11 @@ __global__ __launch_bounds__(G * B * B) void lnt_k_row3(LntParams P) {
     lnt_row3_body<B, G, MODE>(threadIdx.x, blockIdx.x, LView{smem, W}, P);
 }
 
+// 16 words per thread: two workgroups per CU fit the LDS when each wave stays within 128
+// registers (4 waves per SIMD).
 template <int BA, int BB, int BC, int MODE>
-__global__ __launch_bounds__(BB * BC) void lnt_k_row3m(LntParams P) {
+__global__ __launch_bounds__(BB * BC) __attribute__((amdgpu_waves_per_eu(BA == 16 ? 4 : 1))) void lnt_k_row3m(
+    LntParams P) {
     constexpr u32 W = LntRow3m<BA, BB, BC>::LDS;
     __shared__ u64 smem[W];
     lnt_row3m_body<BA, BB, BC, MODE>(threadIdx.x, blockIdx.x, LView{smem, W}, P);
@@ -155,7 +158,8 @@ __global__ __launch_bounds__(G * B * B) void lnt_kb_row3(const LntParams *__rest
 }
 
 template <int BA, int BB, int BC>
-__global__ __launch_bounds__(BB * BC) void lnt_kb_row3m(const LntParams *__restrict__ Ps, u32 wgs) {
+__global__ __launch_bounds__(BB * BC) __attribute__((amdgpu_waves_per_eu(BA == 16 ? 4 : 1))) void lnt_kb_row3m(
+    const LntParams *__restrict__ Ps, u32 wgs) {
     constexpr u32 W = LntRow3m<BA, BB, BC>::LDS;
     __shared__ u64 smem[W];
     const LntParams P = Ps[blockIdx.x / wgs];

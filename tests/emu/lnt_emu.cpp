@@ -311,6 +311,8 @@ static bool run_large_once(Emu *e, bool sub2, const LntParams &P, int mode = LNT
     ok = false;
     if (pl.row3m == 32 && pl.rBB == 16 && pl.rBC == 16) row3m_sq<32, 16, 16>(e, P, mode), ok = true;
     if (pl.row3m == 32 && pl.rBB == 32 && pl.rBC == 16) row3m_sq<32, 32, 16>(e, P, mode), ok = true;
+    if (pl.row3m == 16 && pl.rBB == 32 && pl.rBC == 16) row3m_sq<16, 32, 16>(e, P, mode), ok = true;
+    if (pl.row3m == 16 && pl.rBB == 32 && pl.rBC == 32) row3m_sq<16, 32, 32>(e, P, mode), ok = true;
     if (pl.row3 == 16) row3_sq<16, 1>(e, P, mode), ok = true;
     if (pl.row3 == 8) row3_sq<8, 4>(e, P, mode), ok = true;
 #define RW(b1, b2)                                                        \

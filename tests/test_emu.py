@@ -49,6 +49,20 @@ CASES = [  # (p, n, force_lc): small path 8..4096, large path 2^13..2^17 with se
 ]
 
 
+def test_emulated_8192_word_rows_16_words_per_thread():
+    """8192-word rows as 16 x 32 x 16 (512 threads, half of them idle in phase 2)."""
+    p, n = 2000003, 131072
+    rnd = random.Random(p + 1)
+    o = OracleTest(p, n, True)
+    d = [rnd.getrandbits(wi) for wi in o.widths()]
+    o.set_digits(d)
+    e = Emu(p, n, True, 13, 0, 2)
+    e.set_digits(d)
+    o.iterate(3)
+    e.iterate(3)
+    assert e.digits() == o.digits()
+
+
 @pytest.mark.parametrize("p,n,lc", CASES)
 @pytest.mark.parametrize("prp", [False, True])
 def test_emulated_kernels_match_oracle_random_state(p, n, lc, prp):
@@ -65,14 +79,16 @@ def test_emulated_kernels_match_oracle_random_state(p, n, lc, prp):
     assert e.digits() == o.digits()
 
 
-def test_emulated_16384_word_rows_match_oracle():
-    """Mixed-radix 3-phase rows of 16384 words (32 x 32 x 16) over 16-row columns."""
+@pytest.mark.parametrize("row3", [1, 2])
+def test_emulated_16384_word_rows_match_oracle(row3):
+    """Mixed-radix 3-phase rows of 16384 words over 16-row columns: 32 x 32 x 16 (row3=1)
+    and 16 x 32 x 32 with half the threads idle in phases 2 and 3 (row3=2)."""
     p, n = 4000037, 262144
     rnd = random.Random(p)
     o = OracleTest(p, n, False)
     d = [rnd.getrandbits(wi) for wi in o.widths()]
     o.set_digits(d)
-    e = Emu(p, n, False, 14)
+    e = Emu(p, n, False, 14, 0, row3)
     e.set_digits(d)
     o.iterate(3)
     e.iterate(3)
@@ -135,7 +151,7 @@ def test_emulated_large_path_orbit_matches_oracle():
 
 @pytest.mark.parametrize("p,n,prp,lc", [(521, 32, True, 0), (9689, 512, False, 0), (86243, 4096, True, 0),
                                          (86243, 8192, True, 0), (400009, 32768, False, 0),
-                                         (1000003, 131072, True, 13)])
+                                         (1000003, 131072, True, 13), (1000003, 131072, False, -13)])
 def test_emulated_product_matches_python(p, n, prp, lc):
     """The forward / multiply kernel modes (used by the Gerbicz check): x <- x * y mod M_p
     from pre-carry states with a pending LL -2, then one more squaring."""
@@ -145,7 +161,8 @@ def test_emulated_product_matches_python(p, n, prp, lc):
     rnd = random.Random(n + p)
     M = (1 << p) - 1
     a, b = rnd.randrange(M), rnd.randrange(M)
-    ea, eb = Emu(p, n, prp, lc), Emu(p, n, prp, lc)
+    # lc < 0: split -lc with 16-word mixed-radix rows
+    ea, eb = (Emu(p, n, prp, -lc, 0, 2), Emu(p, n, prp, -lc, 0, 2)) if lc < 0 else (Emu(p, n, prp, lc), Emu(p, n, prp, lc))
     ea.set_digits(digits_of(a, p, n))
     eb.set_digits(digits_of(b, p, n))
     ea.iterate(1)
@@ -162,16 +179,21 @@ def test_emulated_product_matches_python(p, n, prp, lc):
 
 def test_plans_for_the_longest_lengths():
     """n = 2^25 and 2^26 split into 4096-word columns and 8192 / 16384-word rows (mixed-radix
-    3-phase rows 32 x 16 x 16 and 32 x 32 x 16); longer transforms and 2-phase rows of those
-    lengths are rejected."""
+    3-phase rows 16 x 32 x 16 and 32 x 32 x 16 by default); longer transforms and 2-phase
+    rows of those lengths are rejected."""
     L = emu()
     L.emu_plan.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
     out = (ctypes.c_uint32 * 8)()
     assert L.emu_plan(280000001, 1 << 25, 0, -1, out) == 0
+    assert list(out) == [8192, 4096, 0, 16, 32, 16, 16, 4]
+    assert L.emu_plan(280000001, 1 << 25, 0, 1, out) == 0
     assert list(out) == [8192, 4096, 0, 32, 16, 16, 16, 4]
     assert L.emu_plan(560000017, 1 << 26, 0, -1, out) == 0
     assert list(out) == [16384, 4096, 0, 32, 32, 16, 16, 4]
-    assert L.emu_plan(140000041, 1 << 24, 13, -1, out) == 0 and out[3] == 32 and out[1] == 2048
+    assert L.emu_plan(560000017, 1 << 26, 0, 2, out) == 0
+    assert list(out) == [16384, 4096, 0, 16, 32, 32, 16, 4]
+    assert L.emu_plan(140000041, 1 << 24, 13, -1, out) == 0 and out[3] == 16 and out[1] == 2048
+    assert L.emu_plan(140000041, 1 << 24, 0, 2, out) == -1  # 16-word rows only above 4096 words
     assert L.emu_plan(560000017, 1 << 26, 0, 0, out) == -1  # no 2-phase rows of 16384 words
     assert L.emu_plan(1100000017, 1 << 27, 0, -1, out) == -1
     assert L.emu_secured(560000017) <= 26

@@ -97,6 +97,30 @@ def test_longest_lengths_match_oracle(p, n):
     assert np.array_equal(got, o.digits_np())
 
 
+@pytest.mark.parametrize("p,plans", [(637534199, ["row3=1", "row3=2", "lc=14,row3=1", "lc=14,row3=2"]),
+                                     (1207959503, ["row3=1", "row3=2"])])
+def test_longest_length_row_shapes_agree(p, plans, monkeypatch):
+    """Every row shape for 8192 / 16384-word rows (32 or 16 words per thread) and the other
+    split of 2^25 give the same digits as the default plan, from one random state."""
+    import numpy as np
+    from helpers import random_digits_np
+    t = cuda_amd.LucasPRP(p)
+    n = t.n
+    d = random_digits_np(p, n, 11)
+    t.set_digits(d)
+    t.iterate(3)
+    want = np.frombuffer(t.get_digits_buffer(), dtype=np.uint64).copy()
+    t.close()
+    for pl in plans:
+        say("row shape", p, pl)
+        monkeypatch.setenv("LNT_PLAN", pl)
+        t = cuda_amd.LucasPRP(p)
+        t.set_digits(d)
+        t.iterate(3)
+        assert np.array_equal(np.frombuffer(t.get_digits_buffer(), dtype=np.uint64), want), pl
+        t.close()
+
+
 def test_longest_length_squaring_matches_gmp():
     """One LL squaring at n = 2^25, p = 600,000,001, against GMP's mpz product."""
     import numpy as np
