@@ -110,3 +110,30 @@ def test_batched_workload_sample_matches_gmp():
     for p, t in zip(ps, ts):
         assert t.res64() == fixtures[p], p
     batch.close()
+
+
+def test_batch_of_longest_lengths_matches_solo():
+    """Two 2^25 tests (8192-word rows, batched row kernel) and one 2^26 test in a batch."""
+    import numpy as np
+    from helpers import random_digits_np
+    ps = [637534199, 637534157, 1207959503]
+    say("batch longest", ps)
+    batch = cuda_amd.Batch()
+    members = [cuda_amd.LucasPRP(p) for p in ps]
+    starts = [random_digits_np(p, t.n, i) for i, (p, t) in enumerate(zip(ps, members))]
+    for t, d in zip(members, starts):
+        t.set_digits(d)
+        batch.add(t)
+    batch.iterate(4)
+    batch.sync()
+    for t, p, d in zip(members, ps, starts):
+        s = cuda_amd.LucasPRP(p)
+        s.set_digits(d)
+        s.iterate(4)
+        a = np.frombuffer(t.get_digits_buffer(), dtype=np.uint64)
+        b = np.frombuffer(s.get_digits_buffer(), dtype=np.uint64)
+        assert np.array_equal(a, b), p
+        s.close()
+    batch.close()
+    for t in members:
+        t.close()
