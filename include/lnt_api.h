@@ -41,7 +41,10 @@ typedef enum { LNT_MODE_LL = 0, LNT_MODE_PRP = 1 } lnt_mode;
  * Replaces LucasPRP::LucasPRP + LucasPRP::Initialize (lucas_prp.cu:24-111) together
  * with the pool/kernel Initialize() table set-up (kernel.cu:91-153, 199-255) and
  * InitializeTest (lucas_prp.cu:127-134).  The factor split is a hint only: the engine
- * chooses its own row/column split (the role of KernelPool, kernel_pool.cu:17-141). */
+ * chooses its own row/column split (the role of KernelPool, kernel_pool.cu:17-141).
+ * Digit i has width ceil(p(i+1)/n) - ceil(p i/n), computed exactly; the reference's
+ * double-precision form (common/pre_calculation.cu:35) agrees whenever p * n <= 2^53,
+ * which covers every exponent up to n = 2^24. */
 int lnt_create(int device, uint32_t p, const uint32_t *factors, int nfactors, int mode,
                lnt_ctx **out);
 
