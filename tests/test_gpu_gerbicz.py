@@ -112,3 +112,32 @@ def test_gerbicz_c3_size_against_gmp():
         if dres and k in dres:
             assert st["checkpoint_res64"] == dres[k], k
     assert t.gerbicz_stats()["rollbacks"] == 1
+
+
+def test_product_and_gerbicz_at_2p25():
+    """n = 2^25 (8192-word rows in the forward and multiply modes): x * x through lnt_mul
+    against GMP's square, and a checked PRP run with one corruption that is rolled back to
+    the residue of the plain path."""
+    import numpy as np
+    from helpers import gmp_square_digits, random_digits_np
+    p, n = 637534199, 1 << 25
+    say("product 2^25", p)
+    d = random_digits_np(p, n, 3)
+    x, y = cuda_amd.LucasPRP(p, prp=True), cuda_amd.LucasPRP(p, prp=True)
+    x.set_digits(d)
+    y.set_digits(d)
+    x.mul(y)
+    got = np.frombuffer(x.get_digits_buffer(), dtype=np.uint64)
+    x.close()
+    y.close()
+    assert np.array_equal(got, gmp_square_digits(d, p, n, minus_two=False))
+    say("gerbicz 2^25")
+    t = run_checked(p, n, 60, 10, 2, inject_at=17, word=123457, mask=1 << 5)
+    st = t.gerbicz_stats()
+    say(st)
+    assert st["rollbacks"] == 1 and st["verified_iteration"] == 60
+    s = cuda_amd.LucasPRP(p, prp=True)
+    s.iterate(60)
+    assert t.res64() == s.res64()
+    assert np.array_equal(np.frombuffer(t.get_digits_buffer(), dtype=np.uint64),
+                          np.frombuffer(s.get_digits_buffer(), dtype=np.uint64))
