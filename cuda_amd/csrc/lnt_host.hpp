@@ -186,10 +186,9 @@ inline std::string make_plan(uint32_t p, uint64_t n, Plan &pl, PlanOpts opts = P
         // mixed-radix 3-phase rows, one row per workgroup of rBB * rBC threads:
         // row3 = 1: 32 words per thread (8192 = 32*16*16, 16384 = 32*32*16);
         // row3 = 2: 16 words per thread (8192 = 16*32*16, 16384 = 16*32*32).
-        // Default: 16 words at C = 8192 (row kernel 592 -> 540 us at n = 2^25), 32 words
-        // at C = 16384 (the two within 4% there)
+        // Default: 16 words (row kernel at n = 2^25: 594 -> 491 us; at 2^26: 1304 -> 1120 us)
         if (opts.row3 == 0) return "rows above 4096 words need the 3-phase row kernel";
-        const bool w16 = opts.row3 == 2 || (opts.row3 < 0 && pl.C == 8192);
+        const bool w16 = opts.row3 != 1;
         pl.row3m = w16 ? 16 : 32;
         pl.rBB = (pl.C == 8192 && !w16) ? 16 : 32;
         pl.rBC = (pl.C == 16384 && w16) ? 32 : 16;

@@ -576,9 +576,10 @@ LNT_KFN void lnt_row3_body(int tid, int bid, LView lds, const LntParams &P) {
 // 16*32*32), one row per workgroup of A = BB * BC threads holding BA words each.  The
 // phases are those of the 3-phase columns (phase 1 over j of words t + A j; phase 2 items
 // (k1, u), BA * BC of them; phase 3 items (k1, k2), BA * BB of them), with the 4-step
-// twiddles at load and store and the square (or product) in the middle.  With BA = 16 a
-// phase may have fewer items than threads (the rest wait at the barrier): 16 words per
-// thread keep the register count low enough for twice the waves of BA = 32.
+// twiddles at load and store and the square (or product) in the middle.  The phase-2
+// twiddle is applied where phase 3 reads (and its inverse where inverse phase 3 writes).
+// With BA = 16 a phase may have fewer items than threads (the rest wait at the barrier):
+// 16 words per thread keep the register count low enough for twice the waves of BA = 32.
 // LDS word of (k1, a = u + BC v): k1 * RS + v * (BC + 1) + u (padded runs).
 template <int BA, int BB, int BC>
 struct LntRow3m {
@@ -621,21 +622,21 @@ LNT_KFN void lnt_row3m_body(int tid, int bid, LView lds, const LntParams &P) {
             for (int v = 0; v < BB; v++) b[v] = lds[pos(k1, u + BC * v)];
             dft_fwd<BB>(b);
 #pragma unroll
-            for (int rho = 1; rho < BB; rho++) b[rho] = gl_mul(b[rho], P.twr2[(u64)rho * BC + u]);
-#pragma unroll
             for (int rho = 0; rho < BB; rho++) lds[pos(k1, u + BC * brev_c<BB>(rho))] = b[rho];
         }
     }
     LNT_SYNC();
-    // phase 3, the pointwise step, inverse phase 3 (item q of thread t: slots
-    // (q * A + t) * BC + rho, in the order of the items)
+    // phase 3 (with the phase-2 twiddle w_A^(u k2) at the read, where every thread has an
+    // item: phase 2 keeps only its DFT registers live), the pointwise step, inverse phase 3
+    // and the inverse phase-2 twiddle.  Item q of thread t: slots (q * A + t) * BC + rho.
 #pragma unroll
     for (int q = 0; q < Q3; q++) {
         const int I = q * A + t, k1 = I / BB, k2 = I % BB;
         if (N3 >= A || I < N3) {
+            const u64 tb = (u64)brev_c<BB>(k2) * BC;  // twr2 row of k2
             u64 e[BC];
 #pragma unroll
-            for (int u = 0; u < BC; u++) e[u] = lds[pos(k1, u + BC * k2)];
+            for (int u = 0; u < BC; u++) e[u] = gl_mul(lds[pos(k1, u + BC * k2)], P.twr2[tb + u]);
             dft_fwd<BC>(e);
             const u64 so = base + (u64)I * BC;
             if (MODE == LNT_FORWARD) {
@@ -652,22 +653,19 @@ LNT_KFN void lnt_row3m_body(int tid, int bid, LView lds, const LntParams &P) {
             }
             dft_inv<BC>(e);
 #pragma unroll
-            for (int u = 0; u < BC; u++) lds[pos(k1, u + BC * k2)] = e[u];
+            for (int u = 0; u < BC; u++) lds[pos(k1, u + BC * k2)] = gl_mul(e[u], P.itwr2[tb + u]);
         }
     }
     if (MODE == LNT_FORWARD) return;
     LNT_SYNC();
-    // inverse phase 2
+    // inverse phase 2 (its twiddle was applied by inverse phase 3)
 #pragma unroll
     for (int q = 0; q < Q2; q++) {
         const int I = q * A + t, k1 = I / BC, u = I % BC;
         if (N2 >= A || I < N2) {
             u64 b[BB];
 #pragma unroll
-            for (int rho = 0; rho < BB; rho++) {
-                u64 v = lds[pos(k1, u + BC * brev_c<BB>(rho))];
-                b[rho] = rho ? gl_mul(v, P.itwr2[(u64)rho * BC + u]) : v;
-            }
+            for (int rho = 0; rho < BB; rho++) b[rho] = lds[pos(k1, u + BC * brev_c<BB>(rho))];
             dft_inv<BB>(b);
 #pragma unroll
             for (int v = 0; v < BB; v++) lds[pos(k1, u + BC * v)] = b[v];

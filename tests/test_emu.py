@@ -179,7 +179,7 @@ def test_emulated_product_matches_python(p, n, prp, lc):
 
 def test_plans_for_the_longest_lengths():
     """n = 2^25 and 2^26 split into 4096-word columns and 8192 / 16384-word rows (mixed-radix
-    3-phase rows 16 x 32 x 16 and 32 x 32 x 16 by default); longer transforms and 2-phase
+    3-phase rows 16 x 32 x 16 and 16 x 32 x 32 by default); longer transforms and 2-phase
     rows of those lengths are rejected."""
     L = emu()
     L.emu_plan.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
@@ -189,6 +189,8 @@ def test_plans_for_the_longest_lengths():
     assert L.emu_plan(280000001, 1 << 25, 0, 1, out) == 0
     assert list(out) == [8192, 4096, 0, 32, 16, 16, 16, 4]
     assert L.emu_plan(560000017, 1 << 26, 0, -1, out) == 0
+    assert list(out) == [16384, 4096, 0, 16, 32, 32, 16, 4]
+    assert L.emu_plan(560000017, 1 << 26, 0, 1, out) == 0
     assert list(out) == [16384, 4096, 0, 32, 32, 16, 16, 4]
     assert L.emu_plan(560000017, 1 << 26, 0, 2, out) == 0
     assert list(out) == [16384, 4096, 0, 16, 32, 32, 16, 4]
