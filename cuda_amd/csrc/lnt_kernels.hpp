@@ -345,18 +345,19 @@ LNT_KFN void lnt_col3_fwd_body(int tid, int bid, LView lds, const LntParams &P, 
         for (int v = 0; v < BB; v++) b[v] = lds[(u32)((k1 * RS + u + BC * v) * KP + cl)];
         dft_fwd<BB>(b);
 #pragma unroll
-        for (int rho = 1; rho < BB; rho++) b[rho] = gl_mul(b[rho], P.twc2[(u64)rho * BC + u]);
-#pragma unroll
         for (int rho = 0; rho < BB; rho++) lds[(u32)((k1 * RS + BC * brev_c<BB>(rho) + u) * KP + cl)] = b[rho];
     }
     LNT_SYNC();
 
-    // 4. phase 3, store slot t*BA + q*BC + rho (the row pass applies the 4-step twiddle)
+    // 4. phase 3 with the phase-2 twiddle w_A^(u k2) at the read, store slot
+    //    t*BA + q*BC + rho (the row pass applies the 4-step twiddle)
 #pragma unroll
     for (int q = 0; q < S::QC; q++) {
         const int I = q * A + t, k1 = I / BB, k2 = I % BB;
+        const u64 tb = (u64)brev_c<BB>(k2) * BC;
 #pragma unroll
-        for (int u = 0; u < BC; u++) a[q * BC + u] = lds[(u32)((k1 * RS + BC * k2 + u) * KP + cl)];
+        for (int u = 0; u < BC; u++)
+            a[q * BC + u] = gl_mul(lds[(u32)((k1 * RS + BC * k2 + u) * KP + cl)], P.twc2[tb + u]);
         dft_fwd<BC>(a + q * BC);
     }
 #pragma unroll
@@ -375,13 +376,15 @@ LNT_KFN void lnt_col3_inv_body(int tid, int bid, LView lds, const LntParams &P) 
     for (int i = 0; i < BA; i++) a[i] = P.y[c + C * ((u64)t * BA + i)];
 #pragma unroll
     for (int j = 0; j < BA; j++) uv[j] = P.uw[c + C * ((u64)t + (u64)A * j)];
-    // phase 3 inverse
+    // phase 3 inverse, then the inverse phase-2 twiddle
 #pragma unroll
     for (int q = 0; q < S::QC; q++) {
         const int I = q * A + t, k1 = I / BB, k2 = I % BB;
+        const u64 tb = (u64)brev_c<BB>(k2) * BC;
         dft_inv<BC>(a + q * BC);
 #pragma unroll
-        for (int u = 0; u < BC; u++) lds[(u32)((k1 * RS + BC * k2 + u) * KP + cl)] = a[q * BC + u];
+        for (int u = 0; u < BC; u++)
+            lds[(u32)((k1 * RS + BC * k2 + u) * KP + cl)] = gl_mul(a[q * BC + u], P.itwc2[tb + u]);
     }
     LNT_SYNC();
     // phase 2 inverse
@@ -390,10 +393,7 @@ LNT_KFN void lnt_col3_inv_body(int tid, int bid, LView lds, const LntParams &P) 
         const int I = q * A + t, k1 = I / BC, u = I % BC;
         u64 b[BB];
 #pragma unroll
-        for (int rho = 0; rho < BB; rho++) {
-            u64 v = lds[(u32)((k1 * RS + BC * brev_c<BB>(rho) + u) * KP + cl)];
-            b[rho] = rho ? gl_mul(v, P.itwc2[(u64)rho * BC + u]) : v;
-        }
+        for (int rho = 0; rho < BB; rho++) b[rho] = lds[(u32)((k1 * RS + BC * brev_c<BB>(rho) + u) * KP + cl)];
         dft_inv<BB>(b);
 #pragma unroll
         for (int v = 0; v < BB; v++) lds[(u32)((k1 * RS + u + BC * v) * KP + cl)] = b[v];
