@@ -97,9 +97,10 @@ inline uint32_t carry_levels(uint32_t p, uint32_t n) {
 inline bool split_for(uint32_t log_n, uint32_t &lc, uint32_t &lr) {
     if (log_n < 13 || log_n > 26) return false;
     // rows of 4096 words (3-phase row kernel) from n = 2^16 on: measured 1.2-2x faster than
-    // the square-ish splits with 2-phase rows of 256 .. 2048 words; above 2^24 the columns
-    // stay at 4096 words and the rows grow to 8192 / 16384 (mixed-radix 3-phase rows)
-    lc = log_n >= 25 ? log_n - 12 : log_n >= 16 ? 12 : (log_n + 1) / 2;
+    // the square-ish splits with 2-phase rows of 256 .. 2048 words.  From 2^24 the rows
+    // are 8192 words (2^24: 0.550 against 0.581 ms per iteration with 4096 x 4096), and
+    // 16384 at 2^26 (mixed-radix 3-phase rows), over columns of at most 4096 words
+    lc = log_n >= 24 ? (log_n == 26 ? 14 : 13) : log_n >= 16 ? 12 : (log_n + 1) / 2;
     lr = log_n - lc;
     return lr <= 12 && lr >= 4;
 }

@@ -194,8 +194,10 @@ def test_plans_for_the_longest_lengths():
     assert list(out) == [16384, 4096, 0, 32, 32, 16, 16, 4]
     assert L.emu_plan(560000017, 1 << 26, 0, 2, out) == 0
     assert list(out) == [16384, 4096, 0, 16, 32, 32, 16, 4]
-    assert L.emu_plan(140000041, 1 << 24, 13, -1, out) == 0 and out[3] == 16 and out[1] == 2048
-    assert L.emu_plan(140000041, 1 << 24, 0, 2, out) == -1  # 16-word rows only above 4096 words
+    assert L.emu_plan(318767093, 1 << 24, 0, -1, out) == 0
+    assert list(out) == [8192, 2048, 0, 16, 32, 16, 16, 8]  # 2^24: 8192-word rows too
+    assert L.emu_plan(140000041, 1 << 24, 12, -1, out) == 0 and out[0] == 4096 and out[2] == 16
+    assert L.emu_plan(140000041, 1 << 24, 12, 2, out) == -1  # 16-word rows only above 4096 words
     assert L.emu_plan(560000017, 1 << 26, 0, 0, out) == -1  # no 2-phase rows of 16384 words
     assert L.emu_plan(1100000017, 1 << 27, 0, -1, out) == -1
     assert L.emu_secured(560000017) <= 26

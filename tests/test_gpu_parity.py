@@ -75,12 +75,12 @@ def test_large_random_state_matches_oracle(p, n, lc, prp):
     t.close()
 
 
-@pytest.mark.parametrize("p,n", [(600000001, 1 << 25), (1200000001, 1 << 26)])
+@pytest.mark.parametrize("p,n", [(318767093, 1 << 24), (600000001, 1 << 25), (1200000001, 1 << 26)])
 def test_longest_lengths_match_oracle(p, n):
-    """n = 2^25 and 2^26 (rows of 8192 and 16384 words, mixed-radix 3-phase row kernel,
-    over 4096-word 3-phase columns) at exponents near the top of each length: two
-    iterations from a random state, digit for digit against the oracle with exact digit
-    widths (the reference's double-precision widths round at these sizes)."""
+    """n = 2^24, 2^25 and 2^26 (rows of 8192, 8192 and 16384 words, mixed-radix 3-phase row
+    kernel, over 2048- and 4096-word 3-phase columns) at exponents near the top of each
+    length: two iterations from a random state, digit for digit against the oracle with
+    exact digit widths (the reference's double-precision widths round at 2^25 and 2^26)."""
     import numpy as np
     from helpers import random_digits_np
     say("longest lengths", p, n)
