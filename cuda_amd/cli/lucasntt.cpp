@@ -115,6 +115,8 @@ struct Config {
     std::string fft;  // provided length, "" = chosen by the length rule
     int gpu = 0;
     long performanceLoops = 500;
+    long dumpIterations = 0;   // residue dump every this many iterations (0: about hourly)
+    std::string confirmLength; // answer to the length question: "Y", "N" or "" (ask on stdin)
 };
 
 // MainProcess::MainProcess (main_process.cu:64-149).
@@ -137,6 +139,17 @@ static Config ReadConfig(int argc, char **argv) {
     c.dumpResidue = have_ini && ini.GetLongValue("dumpResidue", 0) == 1;
     c.dumpStats = have_ini ? ini.GetLongValue("dumpStats", 1) == 1 : true;
     c.dumpEnvironment = have_ini && ini.GetLongValue("dumpEnvironment", 0) == 1;
+    // keys of this program only: the interval of the safety dump, and a non-interactive
+    // answer to the question asked when the provided length is not the secured one
+    c.dumpIterations = have_ini ? ini.GetLongValue("DumpIterations", 0) : 0;
+    if (have_ini) {
+        std::string a = ini.GetValue("ConfirmLength", "");
+        if (!a.empty()) {
+            char ch = (char)std::toupper((unsigned char)a[0]);
+            if (ch != 'Y' && ch != 'N') throw std::runtime_error("ConfirmLength must be Y or N");
+            c.confirmLength = std::string(1, ch);
+        }
+    }
 
     Ini last;
     c.wasInterrupted = last.LoadFile(c.statFilename) && last.GetLongValue("Interrupted", 0) == 1;
@@ -217,16 +230,35 @@ class LucasPRP {
             if (q == std::string::npos) break;
             pos = q + 1;
         }
-        // time PerformanceLoops iterations of the chosen configuration on scratch state
-        float ms = 0;
-        check(lnt_sync(ctx_, nullptr), "lnt_sync");
-        check(lnt_iterate(ctx_, (uint32_t)cfg_.performanceLoops), "lnt_iterate");
-        check(lnt_sync(ctx_, &ms), "lnt_sync");
-        performance_ms_ = ms;
-        perfLine_ = "Candidate Performance (in ms for " + std::to_string(cfg_.performanceLoops) + " loops): " +
-                    names_[0] + " = " + fmt_g(ms);
+        // every candidate's time, kept like the reference's performance map (name -> ms,
+        // common/common.cu:83-112): console table and "Candidate Performance" stats lines
+        std::vector<char> rep(1 << 16);
+        check(lnt_tune_results(ctx_, rep.data(), rep.size()), "lnt_tune_results");
+        std::vector<std::pair<std::string, double>> perf;
+        std::string text = rep.data();
+        size_t pos2 = 0;
+        while (pos2 < text.size()) {
+            size_t e = text.find('\n', pos2);
+            std::string line = text.substr(pos2, e == std::string::npos ? std::string::npos : e - pos2);
+            pos2 = e == std::string::npos ? text.size() : e + 1;
+            size_t eq = line.rfind(" = ");
+            if (eq == std::string::npos) continue;
+            perf.push_back({line.substr(0, eq), std::atof(line.c_str() + eq + 3)});
+        }
+        std::sort(perf.begin(), perf.end());
+        size_t width = 0;
+        for (auto &pr : perf) width = std::max(width, pr.first.size());
         SAY("\nSummary of the performances of the different processes on %ld loops:", cfg_.performanceLoops);
-        SAY("%s = %g", names_[0].c_str(), ms);
+        perfLines_.clear();
+        for (auto &pr : perf) {
+            SAY("%-*s = %8.3fms", (int)width, pr.first.c_str(), pr.second);
+            perfLines_.push_back("Candidate Performance (in ms for " + std::to_string(cfg_.performanceLoops) +
+                                 " loops): " + pr.first + " = " + fmt_g(pr.second));
+        }
+        // the chosen configuration's time per PerformanceLoops iterations paces the pit stops
+        performance_ms_ = 0;
+        for (auto &pr : perf)
+            if (performance_ms_ == 0 || pr.second < performance_ms_) performance_ms_ = pr.second;
     }
 
     // The tuner's choices persist in LucasNTT.tune (one line per device and length), so a
@@ -326,6 +358,7 @@ class LucasPRP {
         const double perIterMs = performance_ms_ > 0 ? performance_ms_ / cfg_.performanceLoops : 0.05;
         uint64_t pitStop = (uint64_t)(60.0 * 1000.0 / perIterMs * 0.98);
         uint64_t dumpIteration = (uint64_t)(3600.0 * 1000.0 / perIterMs * 0.98);
+        if (cfg_.dumpIterations > 0) dumpIteration = (uint64_t)cfg_.dumpIterations;
         if (pitStop == 0) pitStop = 1;
         if (dumpIteration == 0) dumpIteration = 1;
         const uint64_t batch_cap = 4096;
@@ -342,9 +375,12 @@ class LucasPRP {
                 DumpResidue();
                 DumpStats();
             }
-            // this iteration plus as many following ones as fit before the next event
+            // this iteration plus as many following ones as fit before the next event.  The
+            // dump happens before the iteration whose count j reaches dumpIteration, so the
+            // following iterations of this batch may take j up to dumpIteration - 1 only
+            // (j counts this iteration already and is below dumpIteration here).
             uint64_t left = (uint64_t)cfg_.iterations - currentIteration_ + 1;
-            uint64_t until_dump = dumpIteration - j;  // j counts this iteration already
+            uint64_t until_dump = dumpIteration - 1 - j;
             uint64_t until_pit = pitStop - k - 1;
             uint64_t b = 1 + std::min<uint64_t>({left - 1, until_dump, until_pit, batch_cap - 1});
             check(lnt_iterate(ctx_, (uint32_t)b), "lnt_iterate");
@@ -445,7 +481,7 @@ class LucasPRP {
         std::fprintf(fp, "log2_n = %d\n", l2);
         std::fprintf(fp, "log3_n = %d\n", l3);
         std::fprintf(fp, "log5_n = %d\n", l5);
-        if (!perfLine_.empty()) std::fprintf(fp, "%s\n", perfLine_.c_str());
+        for (auto &l : perfLines_) std::fprintf(fp, "%s\n", l.c_str());
         std::fprintf(fp, "Pool = %s\n", names_[0].c_str());
         for (size_t i = 1; i < names_.size(); i++) std::fprintf(fp, "Kernel%d = %s\n", (int)(i - 1), names_[i].c_str());
         std::fprintf(fp, "Interrupted = %d\n", g_interrupted || currentIteration_ < cfg_.iterations ? 1 : 0);
@@ -465,7 +501,7 @@ class LucasPRP {
     std::vector<uint64_t> host_x_;
     std::vector<Length> lengths_;
     std::vector<std::string> names_;
-    std::string perfLine_;
+    std::vector<std::string> perfLines_;
     double performance_ms_ = 0;
     float totalTime_ = 0;
     uint32_t currentIteration_ = 1;
@@ -481,19 +517,27 @@ int main(int argc, char **argv) {  // main.cpp:22-39
             uint32_t log_n = 0;
             for (auto &l : StringToLengths(cfg.fft)) log_n += l.log2_n;
             int diff = (int)lnt_secured_log2_length(cfg.exponent) - (int)log_n;  // CheckProvidedLength
-            if (diff != 0 &&
-                !Confirm(diff > 0 ? "\n\nThe provided length is too small. Due to the risk of overflow, the result will not be reliable.\nDo you still want to process?"
-                                  : "\n\nThe provided length is not optimal. The process can be accelerated.\nDo you still want to process?"))
-                throw std::runtime_error("Process cancelled by user. Invalid FFT length.");
+            if (diff != 0) {
+                const char *q = diff > 0 ? "\n\nThe provided length is too small. Due to the risk of overflow, the result will not be reliable.\nDo you still want to process?"
+                                         : "\n\nThe provided length is not optimal. The process can be accelerated.\nDo you still want to process?";
+                bool yes;
+                if (cfg.confirmLength.empty()) yes = Confirm(q);
+                else {  // answered by the ConfirmLength key: no prompt, stdin is not read
+                    yes = cfg.confirmLength == "Y";
+                    SAY("%s (Y/N) : %s (ConfirmLength)", q, cfg.confirmLength.c_str());
+                }
+                if (!yes) throw std::runtime_error("Process cancelled by user. Invalid FFT length.");
+            }
         }
         if (std::getenv("LNT_DRY_RUN")) {  // print the resolved configuration only
             std::printf("DRYRUN exponent=%u PRP=%d iterations=%u fromIteration=%u wasInterrupted=%d fft=%s "
                         "dump=%s stats=%s env=%s dumpResidue=%d dumpStats=%d dumpEnvironment=%d gpu=%d loops=%ld "
-                        "pool=%s kernels=%s\n",
+                        "pool=%s kernels=%s dumpIterations=%ld confirmLength=%s\n",
                         cfg.exponent, cfg.PRP, cfg.iterations, cfg.fromIteration, cfg.wasInterrupted, cfg.fft.c_str(),
                         cfg.dumpFilename.c_str(), cfg.statFilename.c_str(), cfg.envFilename.c_str(), cfg.dumpResidue,
                         cfg.dumpStats, cfg.dumpEnvironment, cfg.gpu, cfg.performanceLoops, cfg.preferredPool.c_str(),
-                        cfg.preferredKernels.c_str());
+                        cfg.preferredKernels.c_str(), cfg.dumpIterations,
+                        cfg.confirmLength.empty() ? "ask" : cfg.confirmLength.c_str());
             return 0;
         }
         SAY("");

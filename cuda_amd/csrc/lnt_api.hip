@@ -73,6 +73,7 @@ struct lnt_ctx {
     u64 *f = nullptr;  // transform of the second operand of a product (allocated on first use)
     LntParams P{};
     std::string err;
+    std::string tune_report;  // "name = ms" per candidate of the last lnt_tune
     // Gerbicz-checked PRP (lnt_gerbicz_*): residue slots on the device
     struct Slot {
         u64 *x = nullptr;
@@ -124,100 +125,151 @@ static GView<T> view(T *p, u64 n) {
     return GView<T>{p, n};
 }
 
-// Allocate and fill every table of plan c->pl on c->stream.
-static int build_tables(lnt_ctx *c) {
-    const Plan &pl = c->pl;
+// Word counts of the tables of a plan (0 where the plan has no such table).
+struct TableSizes {
+    u32 n, Lr, L2, R, A;  // residue, row line, second-level row line, column line, second-level column line
+    bool large;
+};
+static TableSizes table_sizes(const Plan &pl) {
+    TableSizes t;
+    t.n = pl.n;
+    t.large = !pl.small;
+    t.Lr = pl.small ? pl.n : pl.C;
+    t.L2 = pl.row3m ? pl.rBB * pl.rBC : pl.row3 ? pl.row3 * pl.row3 : 0;
+    t.R = pl.small ? 0 : pl.R;
+    t.A = (!pl.small && pl.col3) ? pl.R / pl.col3 : 0;
+    return t;
+}
+
+// Device pointers of one set of tables (and the work buffer y).
+struct TablePtrs {
+    u64 *y = nullptr, *w = nullptr, *uw = nullptr, *tw4 = nullptr, *itw4 = nullptr;
+    u64 *twr = nullptr, *itwr = nullptr, *twr2 = nullptr, *itwr2 = nullptr;
+    u64 *twc = nullptr, *itwc = nullptr, *twc2 = nullptr, *itwc2 = nullptr;
+};
+
+// Launch the kernels that fill every table of plan pl (exponent p) on stream s.
+static void fill_tables(const Plan &pl, u32 p, const TablePtrs &t, hipStream_t s) {
+    const TableSizes z = table_sizes(pl);
     const u32 n = pl.n;
-    const size_t nb = (size_t)n * 8;
-    HIPCHK(c, hipMalloc(&c->x, nb), "hipMalloc (x)");
-    HIPCHK(c, hipMalloc(&c->w, nb), "hipMalloc (weights)");
-    HIPCHK(c, hipMalloc(&c->uw, nb), "hipMalloc (unweights)");
-    const u32 Lr = pl.small ? n : pl.C;
-    HIPCHK(c, hipMalloc(&c->twr, (size_t)Lr * 8), "hipMalloc (row twiddles)");
-    HIPCHK(c, hipMalloc(&c->itwr, (size_t)Lr * 8), "hipMalloc (row twiddles)");
-    HIPCHK(c, hipMalloc(&c->y, nb), "hipMalloc (work buffer)");
-    if (!pl.small) {
-        HIPCHK(c, hipMalloc(&c->tw4, nb), "hipMalloc (4-step twiddles)");
-        HIPCHK(c, hipMalloc(&c->itw4, nb), "hipMalloc (4-step twiddles)");
-        HIPCHK(c, hipMalloc(&c->twc, (size_t)pl.R * 8), "hipMalloc (column twiddles)");
-        HIPCHK(c, hipMalloc(&c->itwc, (size_t)pl.R * 8), "hipMalloc (column twiddles)");
-    }
     const u64 Bw = gl_pow(7, (GL_P - 1) / 192 / n * 5);
     const u64 ninv = gl_inv(n);
-    hipLaunchKernelGGL(lnt_k_weights, dim3((n + 255) / 256), dim3(256), 0, c->stream, Bw, gl_inv(Bw), ninv, c->p,
-                       n, c->w, c->uw);
+    hipLaunchKernelGGL(lnt_k_weights, dim3((n + 255) / 256), dim3(256), 0, s, Bw, gl_inv(Bw), ninv, p, n, t.w, t.uw);
+    const u32 Lr = z.Lr;
     u64 omr = gl_pow(554, (GL_P - 1) / Lr);
     // row line twiddles: first phase B1 x B2 (3-phase rows: row3 x row3^2, row3m x rBB rBC)
     const u32 rB1 = pl.row3m ? pl.row3m : pl.row3 ? pl.row3 : pl.row.B1;
     const u32 rB2 = pl.row3m ? pl.rBB * pl.rBC : pl.row3 ? pl.row3 * pl.row3 : pl.row.B2;
-    hipLaunchKernelGGL(lnt_k_line_tw, dim3((Lr + 255) / 256), dim3(256), 0, c->stream, omr, rB1, rB2, Lr, c->twr);
-    hipLaunchKernelGGL(lnt_k_line_tw, dim3((Lr + 255) / 256), dim3(256), 0, c->stream, gl_inv(omr), rB1, rB2, Lr,
-                       c->itwr);
-    if (!pl.small) {
+    hipLaunchKernelGGL(lnt_k_line_tw, dim3((Lr + 255) / 256), dim3(256), 0, s, omr, rB1, rB2, Lr, t.twr);
+    hipLaunchKernelGGL(lnt_k_line_tw, dim3((Lr + 255) / 256), dim3(256), 0, s, gl_inv(omr), rB1, rB2, Lr, t.itwr);
+    if (z.large) {
         u64 omc = gl_pow(554, (GL_P - 1) / pl.R);
         u64 omn = gl_pow(554, (GL_P - 1) / n);
         if (pl.col3) {
             // phase 1 (BA x A line) and phase 2 (BB x BC line) twiddles, 3-phase slot order
-            const u32 A = pl.R / pl.col3;
-            HIPCHK(c, hipMalloc(&c->twc2, (size_t)A * 8), "hipMalloc (column twiddles)");
-            HIPCHK(c, hipMalloc(&c->itwc2, (size_t)A * 8), "hipMalloc (column twiddles)");
-            hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, c->stream, omc, pl.col3, A, pl.R,
-                               c->twc);
-            hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, c->stream, gl_inv(omc), pl.col3,
-                               A, pl.R, c->itwc);
+            const u32 A = z.A;
+            hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, s, omc, pl.col3, A, pl.R, t.twc);
+            hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, s, gl_inv(omc), pl.col3, A, pl.R,
+                               t.itwc);
             u64 oma = gl_pow(554, (GL_P - 1) / A);
-            hipLaunchKernelGGL(lnt_k_line_tw, dim3((A + 255) / 256), dim3(256), 0, c->stream, oma, pl.cBB, pl.cBC, A,
-                               c->twc2);
-            hipLaunchKernelGGL(lnt_k_line_tw, dim3((A + 255) / 256), dim3(256), 0, c->stream, gl_inv(oma), pl.cBB,
-                               pl.cBC, A, c->itwc2);
-            hipLaunchKernelGGL(lnt_k_tw4_3, dim3((n + 255) / 256), dim3(256), 0, c->stream, omn, pl.C, pl.col3, pl.cBB,
-                               pl.cBC, n, c->tw4);
-            hipLaunchKernelGGL(lnt_k_tw4_3, dim3((n + 255) / 256), dim3(256), 0, c->stream, gl_inv(omn), pl.C, pl.col3,
-                               pl.cBB, pl.cBC, n, c->itw4);
+            hipLaunchKernelGGL(lnt_k_line_tw, dim3((A + 255) / 256), dim3(256), 0, s, oma, pl.cBB, pl.cBC, A, t.twc2);
+            hipLaunchKernelGGL(lnt_k_line_tw, dim3((A + 255) / 256), dim3(256), 0, s, gl_inv(oma), pl.cBB, pl.cBC, A,
+                               t.itwc2);
+            hipLaunchKernelGGL(lnt_k_tw4_3, dim3((n + 255) / 256), dim3(256), 0, s, omn, pl.C, pl.col3, pl.cBB, pl.cBC, n,
+                               t.tw4);
+            hipLaunchKernelGGL(lnt_k_tw4_3, dim3((n + 255) / 256), dim3(256), 0, s, gl_inv(omn), pl.C, pl.col3, pl.cBB,
+                               pl.cBC, n, t.itw4);
         } else {
-            hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, c->stream, omc, pl.col.B1,
-                               pl.col.B2, pl.R, c->twc);
-            hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, c->stream, gl_inv(omc),
-                               pl.col.B1, pl.col.B2, pl.R, c->itwc);
-            hipLaunchKernelGGL(lnt_k_tw4, dim3((n + 255) / 256), dim3(256), 0, c->stream, omn, pl.C, pl.col.B1,
-                               pl.col.B2, n, c->tw4);
-            hipLaunchKernelGGL(lnt_k_tw4, dim3((n + 255) / 256), dim3(256), 0, c->stream, gl_inv(omn), pl.C,
-                               pl.col.B1, pl.col.B2, n, c->itw4);
+            hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, s, omc, pl.col.B1, pl.col.B2, pl.R,
+                               t.twc);
+            hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, s, gl_inv(omc), pl.col.B1,
+                               pl.col.B2, pl.R, t.itwc);
+            hipLaunchKernelGGL(lnt_k_tw4, dim3((n + 255) / 256), dim3(256), 0, s, omn, pl.C, pl.col.B1, pl.col.B2, n,
+                               t.tw4);
+            hipLaunchKernelGGL(lnt_k_tw4, dim3((n + 255) / 256), dim3(256), 0, s, gl_inv(omn), pl.C, pl.col.B1,
+                               pl.col.B2, n, t.itw4);
         }
     }
     // second-phase row twiddles: a line of L2 = B2 words split (row3 x row3 or rBB x rBC)
-    const u32 L2 = pl.row3m ? pl.rBB * pl.rBC : pl.row3 ? pl.row3 * pl.row3 : 0;
-    if (L2) {
+    if (z.L2) {
         const u32 s1 = pl.row3m ? pl.rBB : pl.row3, s2 = pl.row3m ? pl.rBC : pl.row3;
-        HIPCHK(c, hipMalloc(&c->twr2, (size_t)L2 * 8), "hipMalloc (row twiddles)");
-        HIPCHK(c, hipMalloc(&c->itwr2, (size_t)L2 * 8), "hipMalloc (row twiddles)");
-        u64 om2 = gl_pow(554, (GL_P - 1) / L2);
-        hipLaunchKernelGGL(lnt_k_line_tw, dim3((L2 + 255) / 256), dim3(256), 0, c->stream, om2, s1, s2, L2, c->twr2);
-        hipLaunchKernelGGL(lnt_k_line_tw, dim3((L2 + 255) / 256), dim3(256), 0, c->stream, gl_inv(om2), s1, s2, L2,
-                           c->itwr2);
+        u64 om2 = gl_pow(554, (GL_P - 1) / z.L2);
+        hipLaunchKernelGGL(lnt_k_line_tw, dim3((z.L2 + 255) / 256), dim3(256), 0, s, om2, s1, s2, z.L2, t.twr2);
+        hipLaunchKernelGGL(lnt_k_line_tw, dim3((z.L2 + 255) / 256), dim3(256), 0, s, gl_inv(om2), s1, s2, z.L2,
+                           t.itwr2);
     }
-    HIPCHK(c, hipGetLastError(), "table kernels launch");
-    LntParams &P = c->P;
-    P.twr2 = view<const u64>(c->twr2, L2);
-    P.itwr2 = view<const u64>(c->itwr2, L2);
-    P.x = view<u64>(c->x, n);
-    P.y = view<u64>(c->y, n);
-    P.w = view<const u64>(c->w, n);
-    P.uw = view<const u64>(c->uw, n);
-    P.tw4 = view<const u64>(c->tw4, pl.small ? 0 : n);
-    P.itw4 = view<const u64>(c->itw4, pl.small ? 0 : n);
-    P.twr = view<const u64>(c->twr, Lr);
-    P.itwr = view<const u64>(c->itwr, Lr);
-    P.twc = view<const u64>(c->twc, pl.small ? 0 : pl.R);
-    P.itwc = view<const u64>(c->itwc, pl.small ? 0 : pl.R);
-    P.twc2 = view<const u64>(c->twc2, pl.col3 ? pl.R / pl.col3 : 0);
-    P.itwc2 = view<const u64>(c->itwc2, pl.col3 ? pl.R / pl.col3 : 0);
-    P.p = c->p;
+}
+
+// Kernel parameters of plan pl over residue x and the tables t.
+static LntParams params_of(const Plan &pl, u32 p, u64 *x, const TablePtrs &t) {
+    const TableSizes z = table_sizes(pl);
+    const u32 n = pl.n;
+    LntParams P{};
+    P.x = view<u64>(x, n);
+    P.y = view<u64>(t.y, n);
+    P.w = view<const u64>(t.w, n);
+    P.uw = view<const u64>(t.uw, n);
+    P.tw4 = view<const u64>(t.tw4, z.large ? n : 0);
+    P.itw4 = view<const u64>(t.itw4, z.large ? n : 0);
+    P.twr = view<const u64>(t.twr, z.Lr);
+    P.itwr = view<const u64>(t.itwr, z.Lr);
+    P.twr2 = view<const u64>(t.twr2, z.L2);
+    P.itwr2 = view<const u64>(t.itwr2, z.L2);
+    P.twc = view<const u64>(t.twc, z.R);
+    P.itwc = view<const u64>(t.itwc, z.R);
+    P.twc2 = view<const u64>(t.twc2, z.A);
+    P.itwc2 = view<const u64>(t.itwc2, z.A);
+    P.p = p;
     P.n = n;
     P.log_n = pl.log_n;
     P.C = pl.C;
     P.R = pl.R;
     P.levels = pl.levels;
+    return P;
+}
+
+// Allocate and fill every table of plan c->pl on c->stream.
+static int build_tables(lnt_ctx *c) {
+    const Plan &pl = c->pl;
+    const TableSizes z = table_sizes(pl);
+    const size_t nb = (size_t)pl.n * 8;
+    HIPCHK(c, hipMalloc(&c->x, nb), "hipMalloc (x)");
+    HIPCHK(c, hipMalloc(&c->y, nb), "hipMalloc (work buffer)");
+    HIPCHK(c, hipMalloc(&c->w, nb), "hipMalloc (weights)");
+    HIPCHK(c, hipMalloc(&c->uw, nb), "hipMalloc (unweights)");
+    HIPCHK(c, hipMalloc(&c->twr, (size_t)z.Lr * 8), "hipMalloc (row twiddles)");
+    HIPCHK(c, hipMalloc(&c->itwr, (size_t)z.Lr * 8), "hipMalloc (row twiddles)");
+    if (z.large) {
+        HIPCHK(c, hipMalloc(&c->tw4, nb), "hipMalloc (4-step twiddles)");
+        HIPCHK(c, hipMalloc(&c->itw4, nb), "hipMalloc (4-step twiddles)");
+        HIPCHK(c, hipMalloc(&c->twc, (size_t)z.R * 8), "hipMalloc (column twiddles)");
+        HIPCHK(c, hipMalloc(&c->itwc, (size_t)z.R * 8), "hipMalloc (column twiddles)");
+    }
+    if (z.A) {
+        HIPCHK(c, hipMalloc(&c->twc2, (size_t)z.A * 8), "hipMalloc (column twiddles)");
+        HIPCHK(c, hipMalloc(&c->itwc2, (size_t)z.A * 8), "hipMalloc (column twiddles)");
+    }
+    if (z.L2) {
+        HIPCHK(c, hipMalloc(&c->twr2, (size_t)z.L2 * 8), "hipMalloc (row twiddles)");
+        HIPCHK(c, hipMalloc(&c->itwr2, (size_t)z.L2 * 8), "hipMalloc (row twiddles)");
+    }
+    TablePtrs t;
+    t.y = c->y;
+    t.w = c->w;
+    t.uw = c->uw;
+    t.tw4 = c->tw4;
+    t.itw4 = c->itw4;
+    t.twr = c->twr;
+    t.itwr = c->itwr;
+    t.twr2 = c->twr2;
+    t.itwr2 = c->itwr2;
+    t.twc = c->twc;
+    t.itwc = c->itwc;
+    t.twc2 = c->twc2;
+    t.itwc2 = c->itwc2;
+    fill_tables(pl, c->p, t, c->stream);
+    HIPCHK(c, hipGetLastError(), "table kernels launch");
+    c->P = params_of(pl, c->p, c->x, t);
     return LNT_OK;
 }
 
@@ -641,6 +693,12 @@ int lnt_describe(const lnt_ctx *c, char *buf, size_t len) {
     return s.size() < len ? LNT_OK : LNT_INVALID_ARGUMENT;
 }
 
+int lnt_tune_results(const lnt_ctx *c, char *buf, size_t len) {
+    if (!c || !buf || !len) return LNT_INVALID_ARGUMENT;
+    std::snprintf(buf, len, "%s", c->tune_report.c_str());
+    return c->tune_report.size() < len ? LNT_OK : LNT_INVALID_ARGUMENT;
+}
+
 int lnt_profile(lnt_ctx *c, uint32_t iters, char *names, size_t names_len, float *ms, int cap) {
     if (!c || !iters) return fail(c, LNT_INVALID_ARGUMENT, "bad argument");
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
@@ -700,7 +758,27 @@ int lnt_profile(lnt_ctx *c, uint32_t iters, char *names, size_t names_len, float
 int lnt_tune(lnt_ctx *c, int loops) {
     if (!c || loops <= 0) return fail(c, LNT_INVALID_ARGUMENT, "bad argument");
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
-    if (c->pl.small) return LNT_OK;  // one configuration only
+    if (c->pl.small) {  // one configuration only: time it
+        float ms = 0;
+        if (lnt_sync(c, nullptr) != LNT_OK) return LNT_DEVICE_ERROR;
+        std::vector<uint64_t> save(c->pl.n);
+        HIPCHK(c, hipMemcpyAsync(save.data(), c->x, (size_t)c->pl.n * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpy");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        const bool pend = c->pending, prist = c->pristine;
+        int rc = run_iterations(c, (uint32_t)loops);
+        if (rc == LNT_OK) rc = lnt_sync(c, &ms);
+        HIPCHK(c, hipMemcpyAsync(c->x, save.data(), (size_t)c->pl.n * 8, hipMemcpyHostToDevice, c->stream), "hipMemcpy");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        c->pending = pend;
+        c->pristine = prist;
+        if (rc != LNT_OK) return rc;
+        std::string nm = describe(c->pl);
+        std::replace(nm.begin(), nm.end(), ';', '/');
+        char line[512];
+        std::snprintf(line, sizeof line, "%s = %g\n", nm.c_str(), ms);
+        c->tune_report = line;
+        return LNT_OK;
+    }
     const u32 n = c->pl.n;
     // current residue, exactly as the next iteration will see it
     std::vector<uint64_t> start(n);
@@ -753,6 +831,16 @@ int lnt_tune(lnt_ctx *c, int loops) {
         cands.push_back(std::move(cd));
     }
     if (cands.empty()) return fail(c, LNT_DEVICE_ERROR, "tuner: no candidate ran");
+    c->tune_report.clear();
+    for (const auto &cd : cands) {
+        Plan cp;
+        make_plan(c->p, n, cp, cd.o);
+        std::string nm = describe(cp);
+        std::replace(nm.begin(), nm.end(), ';', '/');
+        char line[512];
+        std::snprintf(line, sizeof line, "%s = %g\n", nm.c_str(), cd.ms);
+        c->tune_report += line;
+    }
     // every candidate must agree with the others before its time counts
     for (auto &cd : cands)
         if (cd.out != cands[0].out) return fail(c, LNT_DEVICE_ERROR, "tuner: candidates disagree on the residue");
@@ -765,7 +853,7 @@ int lnt_tune(lnt_ctx *c, int loops) {
     }
     Plan bp;
     make_plan(c->p, n, bp, best->o);
-    if (bp.C == c->pl.C && bp.K == c->pl.K && bp.row3 == c->pl.row3 && bp.col3 == c->pl.col3) return LNT_OK;
+    if (same_kernels(bp, c->pl)) return LNT_OK;
     // rebuild this context on the winning plan, keeping the residue
     lnt_ctx *nc = nullptr;
     int rc = make_context(c->device, c->p, n, c->prp, &best->o, &nc);
@@ -796,8 +884,7 @@ int lnt_tune(lnt_ctx *c, int loops) {
 
 static bool same_shape(const lnt_ctx *a, const lnt_ctx *b) {
     // the transform-domain slot order depends on the split and on the row/column variants
-    return a->device == b->device && a->p == b->p && a->prp == b->prp && a->pl.n == b->pl.n && a->pl.C == b->pl.C &&
-           a->pl.small == b->pl.small && a->pl.row3 == b->pl.row3 && a->pl.col3 == b->pl.col3;
+    return a->device == b->device && a->p == b->p && a->prp == b->prp && same_slots(a->pl, b->pl);
 }
 
 int lnt_mul(lnt_ctx *dst, lnt_ctx *src) {
@@ -995,10 +1082,9 @@ static int bfail(lnt_batch *b, int code, const std::string &m) {
                          std::string(what) + " (" + hipGetErrorString(e_) + ")");                         \
     } while (0)
 
-static bool same_group(const Plan &a, const Plan &b) {
-    return a.small == b.small && a.n == b.n && a.C == b.C && a.K == b.K && a.row3 == b.row3 && a.col3 == b.col3 &&
-           a.M == b.M;
-}
+// Members share a batched launch only when every compile-time shape of their kernels and
+// every table layout agree (the carry level count is per member).
+static bool same_group(const Plan &a, const Plan &b) { return same_kernels(a, b); }
 
 static void batch_release(lnt_batch *b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -1062,6 +1148,25 @@ static int batch_build(lnt_batch *b) {
     if (!ok || ec != hipSuccess) return bfail(b, LNT_DEVICE_ERROR, "batch graph capture failed");
     BHIPCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     b->dirty = false;
+    return LNT_OK;
+}
+
+// Each member's "-2" flag from its own state (set_digits, a product or a copy on a member
+// since the last batch step clears it): re-upload the parameters of groups that changed.
+static int batch_refresh_sub2(lnt_batch *b) {
+    for (auto &g : b->groups) {
+        bool changed = false;
+        for (size_t i = 0; i < g.ctxs.size(); i++) {
+            const u32 want = (!g.ctxs[i]->prp && g.ctxs[i]->pending) ? 1u : 0u;
+            if (g.h[i].sub2 != want) {
+                g.h[i].sub2 = want;
+                changed = true;
+            }
+        }
+        if (changed)
+            BHIPCHK(b, hipMemcpyAsync(g.d, g.h.data(), g.h.size() * sizeof(LntParams), hipMemcpyHostToDevice, b->stream),
+                    "hipMemcpy (batch parameters)");
+    }
     return LNT_OK;
 }
 
@@ -1141,6 +1246,7 @@ int lnt_batch_iterate(lnt_batch *b, uint32_t k) {
         for (size_t i = 0; i < g.ctxs.size() && !b->dirty; i++)
             if (g.h[i].x.p != g.ctxs[i]->P.x.p || g.h[i].twr.p != g.ctxs[i]->P.twr.p) b->dirty = true;
     if (b->dirty && (rc = batch_build(b))) return rc;
+    if ((rc = batch_refresh_sub2(b))) return rc;
     // Members under a Gerbicz check count the squarings as checked iterations: the
     // replays stop at each of their block ends for the block step on the member's stream.
     // A batch of single-workgroup tests only runs `step` iterations in one launch per

@@ -65,30 +65,52 @@ struct Plan {
     uint32_t levels = 0;    // carry levels
 };
 
-// Carry levels m such that digits stay bounded by D with 2 n D^2 < P/2 in steady state
-// (see the bound in lnt_kernels.hpp's header).  0 if no m <= 8 works.
+// Two plans run the same kernel instantiations over the same table layouts (only the
+// exponent-dependent carry level count may differ).
+inline bool same_kernels(const Plan &a, const Plan &b) {
+    return a.small == b.small && a.n == b.n && a.C == b.C && a.R == b.R && a.row.B1 == b.row.B1 &&
+           a.row.B2 == b.row.B2 && a.col.B1 == b.col.B1 && a.col.B2 == b.col.B2 && a.K == b.K && a.G == b.G &&
+           a.row3 == b.row3 && a.row3m == b.row3m && a.rBB == b.rBB && a.rBC == b.rBC && a.col3 == b.col3 &&
+           a.cBB == b.cBB && a.cBC == b.cBC && a.M == b.M;
+}
+
+// Two plans order the transform domain the same way (the column tile width, rows per
+// workgroup and carry halo do not change where a frequency is stored).
+inline bool same_slots(const Plan &a, const Plan &b) {
+    Plan x = a;
+    x.K = b.K;
+    x.G = b.G;
+    x.M = b.M;
+    return same_kernels(x, b);
+}
+
+// Carry levels m such that digits stay bounded by D with every pre-carry value below P/2
+// in steady state, in exact integer arithmetic.  0 if no m <= 8 works.
+//  * Digits |d| <= D give convolution outputs |C| <= 2 n D^2 (the IBDWT factors
+//    2^(e_i + e_k - e_j - p t) are 1 or 2), plus 2 for the LL "-2": V_0 = 2 n D^2 + 2,
+//    which must stay <= (P - 1) / 2 to be read back as a signed value.
+//  * A balanced split at width w >= wmin leaves |lo| <= 2^(w-1) and a carry
+//    |h| <= V / 2^w + 1/2, so |h| <= floor(V / 2^wmin) + 1 and one level maps V to
+//    2^(wmax-1) + floor(V / 2^wmin) + 1.
+//  * m levels work when, from D_0 = 2^(wmax-1) + 2, iterating D <- V_m + 1 reaches a D
+//    that V_m + 1 does not exceed (a fixed point of the bound).
 inline uint32_t carry_levels(uint32_t p, uint32_t n) {
-    const double wmin = std::floor((double)p / n), wmax = std::ceil((double)p / n);
-    const double half_p = std::ldexp(1.0, 63) * 0.999999;  // < P/2
+    const uint32_t L = ilog2(n);
+    const uint32_t wmin = p >> L, wmax = (p + n - 1) >> L;
+    if (wmin == 0 || wmax > 62) return 0;
+    const u128 half_p = (u128)((GL_P - 1) / 2);
     for (uint32_t m = 1; m <= 8; m++) {
-        double D = std::ldexp(1.0, (int)wmax - 1) + 2;
-        bool ok = false;
+        u128 D = ((u128)1 << (wmax - 1)) + 2;
         for (int round = 0; round < 200; round++) {
-            double Mv = 2.0 * n * D * D * 1.0001;
-            if (Mv >= half_p) break;
-            double V = Mv + 2;
-            for (uint32_t l = 0; l < m; l++) {
-                double H = V / std::ldexp(1.0, (int)wmin) + 0.5;
-                V = std::ldexp(1.0, (int)wmax - 1) + H;
-            }
-            double Dn = std::ceil(V) + 1;
-            if (Dn <= D) {
-                ok = true;
-                break;
-            }
+            if (D >> 40) break;  // 2 n D^2 would not fit below P/2 anyway
+            const u128 V0 = 2 * (u128)n * D * D + 2;
+            if (V0 > half_p) break;
+            u128 V = V0;
+            for (uint32_t l = 0; l < m; l++) V = ((u128)1 << (wmax - 1)) + (V >> wmin) + 1;
+            const u128 Dn = V + 1;
+            if (Dn <= D) return m;
             D = Dn;
         }
-        if (ok) return m;
     }
     return 0;
 }
@@ -132,8 +154,40 @@ inline bool col_tile_ok(uint32_t R, uint32_t K) {
     return false;
 }
 
-// Build the plan; returns an empty string on success, else the reason.
+// Every size a kernel and its grid assume agrees with the plan's lengths: the launchers
+// refuse a plan that fails this (a kernel of another shape would index out of bounds).
+inline bool plan_consistent(const Plan &pl) {
+    if (pl.n < 8 || (pl.n & (pl.n - 1)) || (1ull << pl.log_n) != pl.n) return false;
+    if (pl.levels < 1 || pl.levels > pl.M || (pl.M != 4 && pl.M != 8)) return false;
+    if (pl.small) {
+        const uint64_t line = pl.row3 ? (uint64_t)pl.row3 * pl.row3 * pl.row3 : (uint64_t)pl.row.B1 * pl.row.B2;
+        return line == pl.n && pl.C == pl.n && pl.R == 1 && !pl.col3 && !pl.row3m;
+    }
+    if ((uint64_t)pl.R * pl.C != pl.n) return false;
+    const uint64_t col = pl.col3 ? (uint64_t)pl.col3 * pl.cBB * pl.cBC : (uint64_t)pl.col.B1 * pl.col.B2;
+    if (col != pl.R) return false;
+    if (!pl.K || pl.C % pl.K) return false;
+    uint64_t row;
+    if (pl.row3m) row = (uint64_t)pl.row3m * pl.rBB * pl.rBC;
+    else if (pl.row3) row = (uint64_t)pl.row3 * pl.row3 * pl.row3;
+    else row = (uint64_t)pl.row.B1 * pl.row.B2;
+    if (row != pl.C) return false;
+    return pl.G && pl.R % pl.G == 0 && (!pl.row3m || pl.G == 1);
+}
+
+inline std::string make_plan_fields(uint32_t p, uint64_t n, Plan &pl, PlanOpts opts);
+
+// Build the plan; returns an empty string on success, else the reason.  pl is rebuilt from
+// scratch on every call (a caller may retry with other options on the same Plan).
 inline std::string make_plan(uint32_t p, uint64_t n, Plan &pl, PlanOpts opts = PlanOpts()) {
+    pl = Plan();
+    std::string why = make_plan_fields(p, n, pl, opts);
+    if (why.empty() && !plan_consistent(pl)) why = "internal error: inconsistent plan";
+    if (!why.empty()) pl = Plan();
+    return why;
+}
+
+inline std::string make_plan_fields(uint32_t p, uint64_t n, Plan &pl, PlanOpts opts) {
     const uint32_t force_lc = opts.lc;
     if (n < 8 || (n & (n - 1))) return "NTT length must be a power of two >= 8";
     pl.n = (uint32_t)n;

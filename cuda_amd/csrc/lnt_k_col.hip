@@ -36,6 +36,7 @@ static bool go_col3_fwd(const lnt::Plan &pl, const LntParams &P, bool sub2, hipS
 }
 
 bool lnt_launch_col_fwd(const lnt::Plan &pl, const LntParams &P, bool sub2, hipStream_t s) {
+    if (!lnt::plan_consistent(pl)) return false;  // a kernel of another shape would run out of bounds
     if (pl.col3) {
 #define C3F(ba, bb, bc, k) \
     if (pl.col3 == ba && pl.cBB == bb && pl.cBC == bc && pl.K == k) return go_col3_fwd<ba, bb, bc, k>(pl, P, sub2, s);
@@ -50,6 +51,7 @@ bool lnt_launch_col_fwd(const lnt::Plan &pl, const LntParams &P, bool sub2, hipS
 }
 
 bool lnt_launch_col_inv(const lnt::Plan &pl, const LntParams &P, hipStream_t s) {
+    if (!lnt::plan_consistent(pl)) return false;  // a kernel of another shape would run out of bounds
     if (pl.col3) {
 #define C3I(ba, bb, bc, k)                                                                           \
     if (pl.col3 == ba && pl.cBB == bb && pl.cBC == bc && pl.K == k) {                                \
@@ -67,6 +69,7 @@ bool lnt_launch_col_inv(const lnt::Plan &pl, const LntParams &P, hipStream_t s) 
 }
 
 bool lnt_launch_batch_col_fwd(const lnt::Plan &pl, const LntParams *Ps, u32 count, hipStream_t s) {
+    if (!lnt::plan_consistent(pl)) return false;  // a kernel of another shape would run out of bounds
     if (pl.col3) {
 #define C3F(ba, bb, bc, k)                                                                                 \
     if (pl.col3 == ba && pl.cBB == bb && pl.cBC == bc && pl.K == k) {                                      \
@@ -96,6 +99,7 @@ bool lnt_launch_batch_col_fwd(const lnt::Plan &pl, const LntParams *Ps, u32 coun
 }
 
 bool lnt_launch_batch_col_inv(const lnt::Plan &pl, const LntParams *Ps, u32 count, hipStream_t s) {
+    if (!lnt::plan_consistent(pl)) return false;  // a kernel of another shape would run out of bounds
     if (pl.col3) {
 #define C3I(ba, bb, bc, k)                                                                                \
     if (pl.col3 == ba && pl.cBB == bb && pl.cBC == bc && pl.K == k) {                                     \

@@ -39,6 +39,7 @@ static bool go_row3m(const lnt::Plan &pl, const LntParams &P, hipStream_t s, int
 }
 
 bool lnt_launch_row(const lnt::Plan &pl, const LntParams &P, hipStream_t s, int mode) {
+    if (!lnt::plan_consistent(pl)) return false;  // a kernel of another shape would run out of bounds
 #define R3M(ba, bb, bc) \
     if (pl.row3m == ba && pl.rBB == bb && pl.rBC == bc) return go_row3m<ba, bb, bc>(pl, P, s, mode);
     R3M(32, 16, 16) R3M(32, 32, 16) R3M(16, 32, 16) R3M(16, 32, 32)
@@ -53,6 +54,7 @@ bool lnt_launch_row(const lnt::Plan &pl, const LntParams &P, hipStream_t s, int 
 }
 
 bool lnt_launch_batch_row(const lnt::Plan &pl, const LntParams *Ps, u32 count, hipStream_t s) {
+    if (!lnt::plan_consistent(pl)) return false;  // a kernel of another shape would run out of bounds
 #define R3M(ba, bb, bc)                                                                                   \
     if (pl.row3m == ba && pl.rBB == bb && pl.rBC == bc) {                                                  \
         hipLaunchKernelGGL((lnt_kb_row3m<ba, bb, bc>), dim3(pl.R * count), dim3(bb * bc), 0, s, Ps, pl.R); \

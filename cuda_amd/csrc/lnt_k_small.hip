@@ -41,6 +41,7 @@ static bool go_small3(const lnt::Plan &pl, const LntParams &P, u32 iters, bool s
 
 bool lnt_launch_small(const lnt::Plan &pl, const LntParams &P, u32 iters, bool sub2_first, bool ll, hipStream_t s,
                       int mode) {
+    if (!lnt::plan_consistent(pl)) return false;  // a kernel of another shape would run out of bounds
     if (pl.row3 == 16) return go_small3<16>(pl, P, iters, sub2_first, ll, s, mode);
     if (pl.row3 == 8) return go_small3<8>(pl, P, iters, sub2_first, ll, s, mode);
     if (pl.row3) return false;
@@ -71,6 +72,7 @@ static bool go_batch_small3(const lnt::Plan &pl, const LntParams *Ps, u32 count,
 }
 
 bool lnt_launch_batch_small(const lnt::Plan &pl, const LntParams *Ps, u32 count, u32 iters, bool ll, hipStream_t s) {
+    if (!lnt::plan_consistent(pl)) return false;  // a kernel of another shape would run out of bounds
     if (pl.row3 == 16) return go_batch_small3<16>(pl, Ps, count, iters, ll, s);
     if (pl.row3 == 8) return go_batch_small3<8>(pl, Ps, count, iters, ll, s);
     if (pl.row3) return false;

@@ -89,7 +89,16 @@ GL_HD void lnt_carry_run(int64_t (&v)[M + K], u32 p, u32 log_n, u64 g0, u32 leve
             e = 0;
         }
     }
-    if (sub2 && g0 == 0) v[M] -= 2;
+    // LL "-2" on digit 0: in its own run, or in the halo of a run starting at g0 <= M
+    // (every window that holds digit 0 must see the same value)
+    if (sub2) {
+        if (g0 == 0) v[M] -= 2;
+        else if (g0 <= (u64)M) {
+#pragma unroll
+            for (int j = 0; j < M; j++)
+                if (j == M - (int)g0) v[j] -= 2;
+        }
+    }
     for (u32 l = 0; l < levels; l++) {
         int64_t hprev = 0;
 #pragma unroll

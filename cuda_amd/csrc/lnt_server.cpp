@@ -62,7 +62,7 @@ uint64_t footprint(uint32_t p, bool gz) {
 struct lnt_server {
     lnt_server_config cfg{};
     std::chrono::steady_clock::time_point t0;
-    std::mutex mu;
+    mutable std::mutex mu;  // guards the queues, the counters and err
     std::condition_variable cv;
     std::deque<Pending> queue;
     std::deque<lnt_result> results;
@@ -73,6 +73,11 @@ struct lnt_server {
     bool stop = false;
     std::string err;
     std::thread worker;
+
+    void set_err(const std::string &m) {
+        std::lock_guard<std::mutex> g(mu);
+        err = m;
+    }
 
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -175,13 +180,13 @@ struct lnt_server {
                 uint64_t k = (uint64_t)std::max(1.0, cfg.step_ms / std::max(ms_per_iter[c], 1e-4));
                 kc[c] = std::min<uint64_t>({k, left[c], 1u << 16});
                 rc = lnt_batch_iterate(batch[c], (uint32_t)kc[c]);
-                if (rc != LNT_OK) err = lnt_batch_last_error(batch[c]);
+                if (rc != LNT_OK) set_err(lnt_batch_last_error(batch[c]));
             }
             for (int c = 0; c < CLASSES && rc == LNT_OK; c++) {
                 if (!kc[c]) continue;
                 float ms = 0;
                 rc = lnt_batch_sync(batch[c], &ms);
-                if (rc != LNT_OK) err = lnt_batch_last_error(batch[c]);
+                if (rc != LNT_OK) set_err(lnt_batch_last_error(batch[c]));
                 ms_per_iter[c] = 0.5 * ms_per_iter[c] + 0.5 * (ms / (double)kc[c]);
                 iterations_run += kc[c];
             }
@@ -287,6 +292,14 @@ void lnt_server_destroy(lnt_server *s) {
     delete s;
 }
 
-const char *lnt_server_last_error(const lnt_server *s) { return s ? s->err.c_str() : ""; }
+// The message is copied under the lock into storage of the calling thread: the pointer
+// stays valid until this thread's next call, whatever the scheduler thread does meanwhile.
+const char *lnt_server_last_error(const lnt_server *s) {
+    thread_local std::string copy;
+    if (!s) return "";
+    std::lock_guard<std::mutex> g(s->mu);
+    copy = s->err;
+    return copy.c_str();
+}
 
 }  // extern "C"

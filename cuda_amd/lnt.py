@@ -36,6 +36,7 @@ EXPORTS = (
     "lnt_batch_destroy", "lnt_batch_last_error", "lnt_server_create", "lnt_server_submit",
     "lnt_server_poll", "lnt_server_stats", "lnt_server_now", "lnt_server_destroy",
     "lnt_server_last_error", "lnt_selftest", "lnt_tune_export", "lnt_tune_import",
+    "lnt_tune_results",
 )
 
 
@@ -83,6 +84,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lnt_res64.argtypes = [vp, u64p]
     lib.lnt_sync.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     lib.lnt_tune.argtypes = [vp, ctypes.c_int]
+    lib.lnt_tune_results.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
     lib.lnt_destroy.argtypes = [vp]
     lib.lnt_destroy.restype = None
     lib.lnt_last_error.argtypes = [vp]
@@ -263,6 +265,16 @@ class LucasPRP:
 
     def tune(self, loops: int = 50) -> None:
         self._check(self._lib.lnt_tune(self._h, int(loops)))
+
+    def tune_results(self) -> List[Tuple[str, float]]:
+        """(candidate, ms for `loops` iterations) of every candidate the last tune() timed."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._check(self._lib.lnt_tune_results(self._h, buf, len(buf)))
+        out = []
+        for line in buf.value.decode().splitlines():
+            name, _, ms = line.rpartition(" = ")
+            out.append((name, float(ms)))
+        return out
 
     # -- products, copies, faults, Gerbicz ------------------------------------------
     def mul(self, other: "LucasPRP") -> None:

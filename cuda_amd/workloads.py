@@ -88,3 +88,21 @@ def served_requests(seed: int = 0, count: int = 2000, rate: float = 4.0,
             p = next_prime(ll_range[0] + int(rng.uniform() * (ll_range[1] - ll_range[0])))
             out.append(Request(i, t, p, False, p - 2))
     return out
+
+
+# Requests of the served mix whose results are pinned by big-integer fixtures: the first
+# twelve LL requests in arrival order (they include p = 52,919, whose length is 2^11), one
+# more LL request of length 2^11, and three PRP chunks (the first two in arrival order and
+# the smallest exponent) cut to their first SAMPLE_PRP_ITERATIONS iterations.
+SAMPLE_PRP_ITERATIONS = 10_000
+
+
+def served_sample(seed: int = 0) -> List[Request]:
+    reqs = served_requests(seed)
+    ll = [r for r in reqs if not r.prp]
+    prp = [r for r in reqs if r.prp]
+    pick = ll[:12]
+    pick += [r for r in ll[12:] if r.p <= 53_248 and r not in pick][:1]
+    chunks = prp[:2] + [min(prp, key=lambda r: r.p)]
+    pick += [Request(r.index, r.arrival, r.p, True, SAMPLE_PRP_ITERATIONS) for r in chunks]
+    return sorted(pick, key=lambda r: r.index)

@@ -72,6 +72,13 @@ int lnt_sync(lnt_ctx *ctx, float *elapsed_ms);
  * (process.cu:45-149), which timed candidates without checking their results. */
 int lnt_tune(lnt_ctx *ctx, int loops);
 
+/* Every candidate timed by the last lnt_tune on ctx, one "name = ms" line each (ms for
+ * `loops` iterations), the names being the lnt_describe strings with '/' between the
+ * parts: the record the reference keeps in its performance map and writes to the stats
+ * file as "Candidate Performance (in ms for N loops): name = t" (common/common.cu:83-85,
+ * 108-112).  Empty before the first lnt_tune. */
+int lnt_tune_results(const lnt_ctx *ctx, char *buf, size_t len);
+
 /* Free everything (LucasPRP::Finalize, lucas_prp.cu:113-125). */
 void lnt_destroy(lnt_ctx *ctx);
 

@@ -384,6 +384,7 @@ extern "C" void emu_set_digits(void *h, const uint64_t *d) {
 
 extern "C" uint32_t emu_levels(void *h) { return ((Emu *)h)->pl.levels; }
 extern "C" uint32_t emu_secured(uint32_t p) { return secured_log2_length(p); }
+extern "C" uint32_t emu_carry_levels(uint32_t p, uint32_t n) { return carry_levels(p, n); }
 
 // The plan make_plan picks (no tables): C, R, row3, row3m, rBB, rBC, col3, K; 0 or -1 if rejected.
 extern "C" int emu_plan(uint32_t p, uint32_t n, int lc, int row3, uint32_t *out) {
@@ -395,6 +396,28 @@ extern "C" int emu_plan(uint32_t p, uint32_t n, int lc, int row3, uint32_t *out)
     const uint32_t v[8] = {pl.C, pl.R, pl.row3, pl.row3m, pl.rBB, pl.rBC, pl.col3, pl.K};
     for (int i = 0; i < 8; i++) out[i] = v[i];
     return 0;
+}
+
+// make_context's plan flow: the given options, and on failure the default plan built into
+// the same Plan object.  out: n, C, R, col3, cBB, cBC, K, row3, row3m, G; returns 1 when the
+// options were used, 0 for the fallback, -1 if both fail; *consistent = plan_consistent.
+extern "C" int emu_plan_retry(uint32_t p, uint32_t n, uint32_t lc, uint32_t K, int row3, int col3, uint32_t *out,
+                              int *consistent) {
+    PlanOpts o;
+    o.lc = lc;
+    o.K = K;
+    o.row3 = row3;
+    o.col3 = col3;
+    Plan pl;
+    int used = 1;
+    if (!make_plan(p, n, pl, o).empty()) {
+        used = 0;
+        if (!make_plan(p, n, pl, PlanOpts()).empty()) return -1;
+    }
+    const uint32_t v[10] = {pl.n, pl.C, pl.R, pl.col3, pl.cBB, pl.cBC, pl.K, pl.row3, pl.row3m, pl.G};
+    for (int i = 0; i < 10; i++) out[i] = v[i];
+    *consistent = plan_consistent(pl) ? 1 : 0;
+    return used;
 }
 
 // dst <- dst * src (same shape), through the forward / multiply kernel modes.

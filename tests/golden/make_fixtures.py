@@ -14,6 +14,10 @@ NTT: LL s_0 = 4, s_k = s_{k-1}^2 - 2 mod M_p; PRP s_0 = 3, s_k = s_{k-1}^2 mod M
         # res64 after 10,000 LL iterations of the 64 batched-workload exponents (seed 0),
         # 4 GMP processes (about an hour); prints "p res64" lines as they finish, then merges
   python tests/golden/make_fixtures.py --import-batched batched.txt
+  python tests/golden/make_fixtures.py --served 4
+        # the served-mix sample (cuda_amd.workloads.served_sample, seed 0): full LL tests
+        # (res64 at every quarter and at the end) and PRP chunks with their Gerbicz
+        # products (block 1000), 4 GMP processes (about 40 minutes)
 
 Output: tests/golden/res64.json
 """
@@ -51,7 +55,8 @@ def next_prime(x: int) -> int:
 
 
 # Prime exponents whose Mersenne numbers are composite (the residue is nonzero).
-COMPOSITES = [p for p in (523, 1009, 2003, 4001, 8191, 9697, 10007, 20011, 44501, 86249)
+# 131 and 307 are the shortest lengths (n = 8 and 16).
+COMPOSITES = [p for p in (131, 307, 523, 1009, 2003, 4001, 8191, 9697, 10007, 20011, 44501, 86249)
               if is_prime(p)]
 
 
@@ -110,6 +115,39 @@ def batched(workers: int, path) -> int:
     return 0
 
 
+def _served_one(req):
+    if req.prp:
+        res, dres = gmp_gerbicz(req.p, req.iterations, 1000)
+        return {"p": req.p, "mode": "PRP", "iters": req.iterations, "block": 1000, "served": True,
+                "res64": res, "dres64": dres}
+    return {"p": req.p, "mode": "LL", "iters": req.iterations, "served": True,
+            "res64": gmp(req.p, "LL", req.iterations, max(1, req.iterations // 4))}
+
+
+def served(workers: int) -> int:
+    """GMP fixtures of the served-mix sample (seed 0)."""
+    sys.path.insert(0, ROOT)
+    from multiprocessing import Pool
+    from cuda_amd.workloads import served_sample
+    reqs = served_sample(0)
+    found = []
+    with Pool(workers) as pool:
+        for case in pool.imap_unordered(_served_one, sorted(reqs, key=lambda r: -r.p)):
+            print(case["p"], case["mode"], case["iters"], flush=True)
+            found.append(case)
+    doc = json.load(open(OUT))
+    keys = {(c["p"], c["mode"], c["iters"]) for c in found}
+    doc["cases"] = [c for c in doc["cases"] if (c["p"], c["mode"], c["iters"]) not in keys or not c.get("served")]
+    have = {(c["p"], c["mode"], c["iters"], c.get("block", 0)) for c in doc["cases"]}
+    for c in sorted(found, key=lambda c: (c["p"], c["mode"])):
+        if (c["p"], c["mode"], c["iters"], c.get("block", 0)) not in have:
+            doc["cases"].append(c)
+    with open(OUT, "w") as f:
+        json.dump(doc, f, indent=1, sort_keys=True)
+    print(f"merged {len(found)} served-sample fixtures", file=sys.stderr)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--long", action="store_true", help="also run the multi-hour p=70M/140M fixtures")
@@ -117,7 +155,10 @@ def main() -> int:
     ap.add_argument("--block", type=int, default=0, help="with --import-log: the log has Gerbicz columns")
     ap.add_argument("--batched", type=int, default=0, metavar="WORKERS")
     ap.add_argument("--import-batched", metavar="FILE")
+    ap.add_argument("--served", type=int, default=0, metavar="WORKERS")
     args = ap.parse_args()
+    if args.served:
+        return served(args.served)
     if args.batched or args.import_batched:
         return batched(args.batched, args.import_batched)
     if args.import_log:
@@ -167,7 +208,7 @@ def main() -> int:
         cases.append({"p": 140000041, "mode": "PRP", "iters": 10000, "block": 1000, "res64": res, "dres64": dres})
     elif prev:
         cases.append(prev[0])
-    cases += [c for c in old["cases"] if c.get("batched")]  # made by --batched (an hour)
+    cases += [c for c in old["cases"] if c.get("batched") or c.get("served")]  # --batched, --served
     doc = {
         "generator": "tests/golden/make_fixtures.py (oracle/gmp_ll.c, GMP mpz)",
         "definition": "LL: s0=4, s=s^2-2 mod 2^p-1; PRP: s0=3, s=s^2 mod 2^p-1; res64 = low 64 bits "

@@ -16,8 +16,10 @@ def run(args, cwd, env=None, stdin=None, timeout=600):
     e = dict(os.environ)
     if env:
         e.update(env)
-    return subprocess.run([CLI] + [str(a) for a in args], cwd=cwd, env=e, input=stdin,
-                          capture_output=True, text=True, timeout=timeout)
+    # stdin=None: a closed stdin (end of input at once)
+    kw = {"input": stdin} if stdin is not None else {"stdin": subprocess.DEVNULL}
+    return subprocess.run([CLI] + [str(a) for a in args], cwd=cwd, env=e, capture_output=True,
+                          text=True, timeout=timeout, **kw)
 
 
 def dry(args, cwd, stdin=""):
@@ -68,6 +70,27 @@ def test_non_secured_length_asks_for_confirmation(tmp_path):
     assert r.returncode == 0 and c["fft"] == "8K"
     r, c = dry([86243, 100, "x", "4K"], tmp_path)
     assert r.returncode == 0
+
+
+def test_confirm_length_key_answers_without_stdin(tmp_path):
+    """ConfirmLength answers the length question of main_process.cu:137-141 for scripted
+    runs: stdin is never read (it is closed here), Y proceeds, N cancels."""
+    (tmp_path / "LucasNTT.ini").write_text("ConfirmLength = Y\n")
+    r, c = dry([86243, 100, "x", "8K"], tmp_path, stdin=None)
+    assert r.returncode == 0 and c["fft"] == "8K" and c["confirmLength"] == "Y"
+    assert "not optimal" in r.stdout and "(ConfirmLength)" in r.stdout
+    (tmp_path / "LucasNTT.ini").write_text("ConfirmLength = no\n")
+    r, c = dry([86243, 100, "x", "2K"], tmp_path, stdin=None)
+    assert r.returncode == 1 and "Process cancelled by user" in r.stdout
+    (tmp_path / "LucasNTT.ini").write_text("ConfirmLength = maybe\n")
+    r, c = dry([86243, 100, "x", "2K"], tmp_path, stdin=None)
+    assert r.returncode == 1 and "ConfirmLength must be Y or N" in r.stdout
+    # without the key, a closed stdin still means "N"
+    (tmp_path / "LucasNTT.ini").write_text("DumpIterations = 250\n")
+    r, c = dry([86243, 100, "x", "8K"], tmp_path, stdin=None)
+    assert r.returncode == 1 and "Process cancelled by user" in r.stdout
+    r, c = dry([], tmp_path)
+    assert c["dumpIterations"] == "250" and c["confirmLength"] == "ask"
 
 
 def test_resume_reads_the_stats_file(tmp_path):
@@ -129,6 +152,28 @@ def test_partial_run_dumps_the_reference_residue_format(tmp_path):
     assert [int(l, 16) for l in lines] == o.digits()
     stats = (tmp_path / "part.txt").read_text()
     assert "Conclusion = Partial" in stats and f"Res64 = {golden(86243, 'LL')[1000]:016X}" in stats
+
+
+@pytest.mark.gpu
+def test_safety_dump_every_dump_iterations(tmp_path):
+    """The periodic safety dump (lucas_prp.cu:207-214) runs before the iteration whose
+    count reaches the interval: with DumpIterations = 1000 a full LL of M9689 (9,687
+    iterations) dumps 9 times, the last one the residue after 8,999 iterations, with the
+    stats file of that moment.  A full run writes no final .dat (and dumpStats = 0 no
+    final stats), so the files left are the last safety dump's."""
+    from cuda_amd import secured_log2_length
+    (tmp_path / "LucasNTT.ini").write_text("PerformanceLoops = 0\nDumpIterations = 1000\n"
+                                           "dumpStats = 0\n")
+    r = run([9689, 0, "d"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "M(9689) is prime!" in r.stdout
+    assert r.stdout.count("Now dumping intermediate residue for safety reasons") == 9
+    lines = (tmp_path / "d.dat").read_text().splitlines()
+    o = OracleTest(9689, 1 << secured_log2_length(9689))
+    o.iterate(8999)
+    assert [int(l, 16) for l in lines] == o.digits()
+    stats = (tmp_path / "d.txt").read_text()
+    assert "currentIteration = 9000" in stats and "Interrupted = 1" in stats
 
 
 @pytest.mark.gpu

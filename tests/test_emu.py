@@ -201,3 +201,112 @@ def test_plans_for_the_longest_lengths():
     assert L.emu_plan(560000017, 1 << 26, 0, 0, out) == -1  # no 2-phase rows of 16384 words
     assert L.emu_plan(1100000017, 1 << 27, 0, -1, out) == -1
     assert L.emu_secured(560000017) <= 26
+
+
+def _levels_py(p, n):
+    """The carry bound of lnt_host.hpp (carry_levels), restated with Python integers."""
+    L = n.bit_length() - 1
+    wmin, wmax = p >> L, (p + n - 1) >> L
+    half_p = ((1 << 64) - (1 << 32)) // 2
+    for m in range(1, 9):
+        D = (1 << (wmax - 1)) + 2
+        for _ in range(200):
+            V0 = 2 * n * D * D + 2
+            if V0 > half_p:
+                break
+            V = V0
+            for _ in range(m):
+                V = (1 << (wmax - 1)) + (V >> wmin) + 1
+            if V + 1 <= D:
+                return m
+            D = V + 1
+    return 0
+
+
+def _simulate_worst_case(p, n, m):
+    """Brute-force check of the bound on digit magnitudes: the largest pre-carry values the
+    bound allows, pushed through m balanced levels with the worst-case carries (all of
+    the same sign), must come back within the fixed point."""
+    L = n.bit_length() - 1
+    wmin, wmax = p >> L, (p + n - 1) >> L
+    D = (1 << (wmax - 1)) + 2
+    for _ in range(200):
+        V = 2 * n * D * D + 2
+        vals = [V] * (m + 1)
+        for _ in range(m):
+            out = [0] * len(vals)
+            for j, v in enumerate(vals):
+                w = wmin
+                h = (v + (1 << (w - 1))) >> w
+                lo = v - (h << w)
+                out[j] += lo
+                if j + 1 < len(out):
+                    out[j + 1] += h
+            vals = out
+        Dn = max(abs(v) for v in vals[m:]) + (1 << (wmax - 1))
+        if Dn <= D:
+            return D
+        D = Dn
+    return None
+
+
+def test_carry_levels_exact_at_the_top_exponent_of_every_length():
+    """For every length 2^3 .. 2^26, the largest exponent the length rule assigns to it and
+    a sample below: the engine's carry level count (exact integers) equals the Python
+    restatement, is at least 1, keeps 2 n D^2 + 2 <= (P-1)/2, and a brute-force push of the
+    worst-case values through that many levels stays within the bound."""
+    L = emu()
+    L.emu_carry_levels.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+    L.emu_carry_levels.restype = ctypes.c_uint32
+    L.emu_secured.argtypes = [ctypes.c_uint32]
+    L.emu_secured.restype = ctypes.c_uint32
+    for log_n in range(3, 27):
+        n = 1 << log_n
+        # largest p with secured length log_n (bisection on the monotone length rule)
+        lo, hi = 33, 2 ** 32 - 1
+        while lo < hi:
+            mid = (lo + hi + 1) // 2
+            if L.emu_secured(mid) <= log_n:
+                lo = mid
+            else:
+                hi = mid - 1
+        top = lo
+        assert L.emu_secured(top) == log_n
+        for p in (top, top - 1, top - 1000, (top * 3) // 4):
+            if p < 2 * n or L.emu_secured(p) != log_n:
+                continue
+            m = L.emu_carry_levels(p, n)
+            assert m == _levels_py(p, n), (p, n)
+            assert 1 <= m <= 8, (p, n)
+            assert _simulate_worst_case(p, n, m) is not None, (p, n, m)
+
+
+def test_plan_fallback_is_the_default_plan():
+    """A context whose options (LNT_PLAN or a tuned entry) do not fit the length falls back
+    to the default plan built into the same Plan object: every field must be the default's
+    (a stale field once kept a 1024-row column kernel on 32-row columns), and every plan
+    must pass the launchers' consistency check."""
+    L = emu()
+    L.emu_plan_retry.argtypes = [ctypes.c_uint32] * 6 + [ctypes.c_void_p, ctypes.c_void_p]
+    L.emu_plan_retry.restype = ctypes.c_int
+    L.emu_plan_retry.argtypes[4] = ctypes.c_int
+    L.emu_plan_retry.argtypes[5] = ctypes.c_int
+    L.emu_secured.argtypes = [ctypes.c_uint32]
+    L.emu_secured.restype = ctypes.c_uint32
+    out = (ctypes.c_uint32 * 10)()
+    ok = ctypes.c_int()
+    for log_n in range(13, 27):
+        n = 1 << log_n
+        p = 2 * n + 1
+        while L.emu_secured(p) < log_n or any(p % q == 0 for q in range(2, 2000) if q * q <= p):
+            p += 2
+        assert L.emu_plan_retry(p, n, 0, 0, -1, -1, out, ctypes.byref(ok)) == 1
+        default = tuple(out)
+        for lc in range(7, 15):
+            for K in (16, 8, 4):
+                for r3 in (0, 1, 2):
+                    for c3 in (0, 1):
+                        used = L.emu_plan_retry(p, n, lc, K, r3, c3, out, ctypes.byref(ok))
+                        assert used in (0, 1) and ok.value == 1, (n, lc, K, r3, c3)
+                        if used == 0:
+                            assert tuple(out) == default, (n, lc, K, r3, c3, tuple(out), default)

@@ -137,3 +137,68 @@ def test_batch_of_longest_lengths_matches_solo():
     batch.close()
     for t in members:
         t.close()
+
+
+def test_members_of_different_row_shapes_are_not_grouped(monkeypatch):
+    """At 2^24 and 2^25 the row kernel has 32- and 16-words-per-thread variants (row3=1 and
+    row3=2) with different table layouts; members of each shape, in one batch, must each
+    equal their solo run."""
+    import numpy as np
+    from helpers import random_digits_np
+    cases = [(318767093, "row3=1"), (318767093, "row3=2"), (637534199, "row3=1"), (637534199, "row3=2")]
+    batch = cuda_amd.Batch()
+    members, starts = [], []
+    for i, (p, plan) in enumerate(cases):
+        monkeypatch.setenv("LNT_PLAN", plan)
+        t = cuda_amd.LucasPRP(p)
+        say("member", p, plan, t.describe())
+        d = random_digits_np(p, t.n, 100 + i)
+        t.set_digits(d)
+        batch.add(t)
+        members.append(t)
+        starts.append(d)
+    monkeypatch.delenv("LNT_PLAN")
+    assert len({tuple(t.describe()) for t in members}) == 4
+    batch.iterate(3)
+    batch.sync()
+    for (p, plan), t, d in zip(cases, members, starts):
+        s = cuda_amd.LucasPRP(p)  # default plan, run alone
+        s.set_digits(d)
+        s.iterate(3)
+        a = np.frombuffer(t.get_digits_buffer(), dtype=np.uint64)
+        b = np.frombuffer(s.get_digits_buffer(), dtype=np.uint64)
+        assert np.array_equal(a, b), (p, plan)
+        s.close()
+    batch.close()
+    for t in members:
+        t.close()
+
+
+def test_member_state_changed_between_batch_steps():
+    """set_digits, a product and a copy on LL members already in a batch clear their
+    pending "-2"; the next batch step must not subtract it (each equals a solo run)."""
+    p = 1000003
+    batch = cuda_amd.Batch()
+    a, b, c = cuda_amd.LucasPRP(p), cuda_amd.LucasPRP(p), cuda_amd.LucasPRP(p)
+    sa, sb, sc = cuda_amd.LucasPRP(p), cuda_amd.LucasPRP(p), cuda_amd.LucasPRP(p)
+    for t in (a, b, c):
+        batch.add(t)
+    batch.iterate(5)
+    for s in (sa, sb, sc):
+        s.iterate(5)
+    batch.sync()
+    d = sa.get_digits()
+    d[3] ^= 1
+    a.set_digits(d)
+    sa.set_digits(d)
+    b.mul(b)
+    sb.mul(sb)
+    c.copy_from(a)
+    sc.copy_from(sa)
+    batch.iterate(4)
+    for s in (sa, sb, sc):
+        s.iterate(4)
+    batch.sync()
+    for t, s, what in ((a, sa, "set_digits"), (b, sb, "mul"), (c, sc, "copy")):
+        assert t.get_digits() == s.get_digits(), what
+    batch.close()

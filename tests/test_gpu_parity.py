@@ -20,7 +20,9 @@ def say(*a):
     sys.stdout.flush()
 
 
-SMALL_KAT = [c for c in golden_cases() if c["p"] <= 23209]
+# every fixture exponent up to 44,501: lengths 8 (p = 131), 16 (p = 307) ... 2048 (p = 44,497
+# and 44,501), the single-workgroup kernels of every length below 4096
+SMALL_KAT = [c for c in golden_cases() if c["p"] <= 44501 and "block" not in c]
 
 
 @pytest.mark.parametrize("case", SMALL_KAT, ids=lambda c: f"{c['mode']}{c['p']}")
@@ -52,13 +54,13 @@ def test_m86243_full_ll_every_10000():
     assert set(d) == {0}
 
 
-@pytest.mark.parametrize("p,n,lc", [
-    (86243, 8192, 0), (200003, 16384, 0), (400009, 32768, 0), (1000003, 65536, 0),
-    (2000003, 131072, 0), (4000037, 262144, 0), (8000009, 524288, 0), (16000057, 1 << 20, 0),
-    (33000011, 1 << 21, 0), (70000027, 1 << 22, 0),
+@pytest.mark.parametrize("p,n", [
+    (86243, 8192), (200003, 16384), (400009, 32768), (1000003, 65536),
+    (2000003, 131072), (4000037, 262144), (8000009, 524288), (16000057, 1 << 20),
+    (33000011, 1 << 21), (70000027, 1 << 22),
 ])
 @pytest.mark.parametrize("prp", [False, True])
-def test_large_random_state_matches_oracle(p, n, lc, prp):
+def test_large_random_state_matches_oracle(p, n, prp):
     say("large random", p, n, prp)
     rnd = random.Random(p + n + prp)
     o = OracleTest(p, n, prp, n0=1 if n <= 65536 else 1024)
@@ -73,6 +75,44 @@ def test_large_random_state_matches_oracle(p, n, lc, prp):
     t.iterate(k)
     assert t.get_digits() == o.digits()
     t.close()
+
+
+# The tuner's candidate grid (lnt_tune): row length 2^lc, column tile K, row and column
+# variants.  Shapes a length does not support fall back to its default plan, so the
+# distinct plans are those lnt_describe reports.
+TUNER_GRID = [f"lc={lc},K={K},row3={r3},col3={c3}" for lc in range(7, 15) for K in (16, 8, 4)
+              for r3 in (0, 1, 2) for c3 in (0, 1)]
+
+
+@pytest.mark.parametrize("p,n", [(2000003, 1 << 17), (16000057, 1 << 20), (70000027, 1 << 22),
+                                 (140000041, 1 << 23)])
+def test_every_tunable_plan_matches_oracle(p, n, monkeypatch):
+    """Every plan lnt_tune can choose at this length, forced through LNT_PLAN, against the
+    oracle digit for digit from a random residue (LL, two iterations)."""
+    import numpy as np
+    from helpers import random_digits_np
+    d = random_digits_np(p, n, seed=n + 5)
+    o = OracleTest(p, n, False, n0=1024)
+    o.set_digits_np(d)
+    o.iterate(2)
+    want = o.digits_np()
+    seen = set()
+    for plan in TUNER_GRID:
+        monkeypatch.setenv("LNT_PLAN", plan)
+        t = cuda_amd.LucasPRP(p, n)
+        desc = tuple(t.describe())
+        if desc in seen:
+            t.close()
+            continue
+        seen.add(desc)
+        say("plan", n, plan, ";".join(desc))
+        t.set_digits(d.tolist())
+        t.iterate(2)
+        got = np.asarray(t.get_digits_buffer(), dtype=np.uint64)
+        assert np.array_equal(got, want), (n, plan, desc)
+        t.close()
+    say("distinct plans", n, len(seen))
+    assert len(seen) >= 3
 
 
 @pytest.mark.parametrize("p,n", [(318767093, 1 << 24), (600000001, 1 << 25), (1200000001, 1 << 26)])
