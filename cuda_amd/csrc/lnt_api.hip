@@ -1319,3 +1319,120 @@ void lnt_batch_destroy(lnt_batch *b) {
 const char *lnt_batch_last_error(const lnt_batch *b) { return b ? b->err.c_str() : lnt_last_error(nullptr); }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ iterations without allocation
+// lnt_iterate_static: the squaring of lnt_iterate on a residue in the caller's device memory,
+// with the work buffer and every table in `scratch`, a fixed array of this code object
+// (one copy per device, held until the process ends).  It allocates no device memory.
+// Residues of up to LNT_STATIC_MAX_N words: five n-word buffers (work buffer, weights,
+// unweights, 4-step twiddles and their inverses) plus the line twiddles.
+static constexpr u64 kStaticMaxN = LNT_STATIC_MAX_N;
+static constexpr u64 kStaticWords = 5 * kStaticMaxN + 65536;
+static constexpr int kStaticDevices = 64;
+static __device__ __align__(256) u64 scratch[kStaticWords];
+
+// x[0] <- x[0] - 2 (field encoding of a signed value): the LL "-2" of the last iteration.
+__global__ void lnt_k_minus2(u64 *x) { x[0] = gl_sub(x[0], 2); }
+
+namespace {
+struct StaticDevice {
+    u64 *base = nullptr;  // device address of `scratch` on this device (hipGetSymbolAddress)
+    bool tables = false;  // tables of (p, n) below are in scratch
+    u32 p = 0, n = 0;
+};
+std::mutex g_static_mu;
+StaticDevice g_static[kStaticDevices];
+
+// Carve the buffers of plan pl out of scratch (each 256-byte aligned); false if they do
+// not fit.
+bool static_layout(const Plan &pl, u64 *base, TablePtrs &t) {
+    const TableSizes z = table_sizes(pl);
+    u64 off = 0;
+    auto take = [&](u64 words) {
+        u64 *ptr = base + off;
+        off += (words + 31) & ~(u64)31;
+        return ptr;
+    };
+    t.y = take(z.n);
+    t.w = take(z.n);
+    t.uw = take(z.n);
+    t.twr = take(z.Lr);
+    t.itwr = take(z.Lr);
+    if (z.large) {
+        t.tw4 = take(z.n);
+        t.itw4 = take(z.n);
+        t.twc = take(z.R);
+        t.itwc = take(z.R);
+    }
+    if (z.A) {
+        t.twc2 = take(z.A);
+        t.itwc2 = take(z.A);
+    }
+    if (z.L2) {
+        t.twr2 = take(z.L2);
+        t.itwr2 = take(z.L2);
+    }
+    return off <= kStaticWords;
+}
+}  // namespace
+
+extern "C" int lnt_iterate_static(uint32_t p, int mode, uint64_t *x, size_t n, uint32_t k, void *stream) {
+    if (mode != LNT_MODE_LL && mode != LNT_MODE_PRP) return fail(nullptr, LNT_INVALID_ARGUMENT, "bad mode");
+    if (!x) return fail(nullptr, LNT_INVALID_ARGUMENT, "lnt_iterate_static: x is NULL");
+    if (n > kStaticMaxN)
+        return fail(nullptr, LNT_INVALID_ARGUMENT,
+                    "lnt_iterate_static: n exceeds the static maximum of " + std::to_string(kStaticMaxN) + " words");
+    Plan pl;
+    std::string why = make_plan(p, n, pl, PlanOpts());
+    if (!why.empty()) return fail(nullptr, LNT_UNSUPPORTED, "lnt_iterate_static: " + why);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kStaticDevices)
+        return fail(nullptr, LNT_DEVICE_ERROR, "lnt_iterate_static: no current device");
+    hipStream_t s = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    StaticDevice &sd = g_static[dev];
+    if (!sd.base) {
+        // the array's host-side name is not a device address: take this device's copy
+        u64 *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(nullptr, LNT_DEVICE_ERROR,
+                        std::string("lnt_iterate_static: hipGetSymbolAddress failed (") + hipGetErrorString(e) + ")");
+        sd.base = ptr;
+    }
+    TablePtrs t;
+    if (!static_layout(pl, sd.base, t))
+        return fail(nullptr, LNT_UNSUPPORTED, "lnt_iterate_static: the plan's tables exceed the static scratch");
+    if (!sd.tables || sd.p != p || sd.n != (u32)n) {
+        fill_tables(pl, p, t, s);
+        sd.tables = true;
+        sd.p = p;
+        sd.n = (u32)n;
+    }
+    if (k == 0) return LNT_OK;
+    const bool ll = mode == LNT_MODE_LL;
+    const LntParams P = params_of(pl, p, x, t);
+    bool ok = true;
+    if (pl.small) {
+        const uint32_t chunk = pl.n >= 2048 ? 256 : 1024;
+        bool first = true;
+        for (uint32_t left = k; left && ok;) {
+            const uint32_t c = left < chunk ? left : chunk;
+            ok = lnt_launch_small(pl, P, c, ll && !first, ll, s);
+            first = false;
+            left -= c;
+        }
+    } else {
+        for (uint32_t i = 0; i < k && ok; i++)
+            ok = lnt_launch_col_fwd(pl, P, ll && i > 0, s) && lnt_launch_row(pl, P, s) && lnt_launch_col_inv(pl, P, s);
+    }
+    if (!ok) return fail(nullptr, LNT_UNSUPPORTED, "lnt_iterate_static: no kernel for this length");
+    if (ll) hipLaunchKernelGGL(lnt_k_minus2, dim3(1), dim3(1), 0, s, x);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        sd.tables = false;
+        return fail(nullptr, LNT_DEVICE_ERROR,
+                    std::string("lnt_iterate_static: kernel launch (") + hipGetErrorString(e) + ")");
+    }
+    return LNT_OK;
+}

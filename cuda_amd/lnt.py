@@ -36,7 +36,7 @@ EXPORTS = (
     "lnt_batch_destroy", "lnt_batch_last_error", "lnt_server_create", "lnt_server_submit",
     "lnt_server_poll", "lnt_server_stats", "lnt_server_now", "lnt_server_destroy",
     "lnt_server_last_error", "lnt_selftest", "lnt_tune_export", "lnt_tune_import",
-    "lnt_tune_results",
+    "lnt_tune_results", "lnt_iterate_static",
 )
 
 

@@ -137,6 +137,26 @@ int lnt_gerbicz_iterate(lnt_ctx *ctx, uint32_t k);
  * [3] rollbacks, [4] res64 of D at the last verified checkpoint.  cap >= 5. */
 int lnt_gerbicz_stats(lnt_ctx *ctx, uint64_t *out, int cap);
 
+/* ---- iterations without device allocation ----
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * k iterations (square, and -2 for LL) of a residue held in the caller's device memory,
+ * enqueued on `stream` (a hipStream_t; NULL = the null stream) on the current device.
+ * x holds n words, each a field element of GF(2^64 - 2^32 + 1) encoding a signed digit
+ * value v_i (the word itself if <= (P-1)/2, else word - P); the residue is
+ * sum v_i 2^ceil(p i/n) mod 2^p - 1.  Canonical natural digits (the .dat order) are such
+ * a vector, and so is the output, which holds the k-th residue after the call; it is a
+ * valid input of the next call.  n is a power of two the engine supports with
+ * n <= LNT_STATIC_MAX_N; larger n fails with LNT_INVALID_ARGUMENT.
+ * The work buffer and the tables live in a fixed device array of the library
+ * (5 * LNT_STATIC_MAX_N + 65536 words, 40.5 MiB, one copy per device, held until the
+ * process ends); it keeps the tables of the last (p, n) and rebuilds them when they
+ * change.  Calls on one device share that array: calls that overlap in time on a device
+ * (other streams, other threads) race, so order them on one stream.  Asynchronous like
+ * lnt_iterate; errors are reported by lnt_last_error(NULL). */
+#define LNT_STATIC_MAX_N (1u << 20)
+int lnt_iterate_static(uint32_t p, int mode, uint64_t *x, size_t n, uint32_t k, void *stream);
+
 /* ---- batches of independent tests (new: the reference runs one test per process) ----
  * Members keep their own residues; members of the same shape run as one batched launch
  * per kernel and one iteration of all members is a single hipGraph launch.  Work issued
