@@ -107,6 +107,7 @@ struct lnt_server {
                 if (lnt_gerbicz_stats(a.ctx, st, 5) == LNT_OK) {
                     r.checks = st[2];
                     r.rollbacks = st[3];
+                    r.check_res64 = st[4];
                 }
             }
             r.status = status;

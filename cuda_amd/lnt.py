@@ -63,7 +63,8 @@ class Result(ctypes.Structure):
     _fields_ = [("tag", ctypes.c_uint64), ("p", ctypes.c_uint32), ("prp", ctypes.c_int32),
                 ("iterations", ctypes.c_uint64), ("res64", ctypes.c_uint64), ("is_zero", ctypes.c_int32),
                 ("status", ctypes.c_int32), ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double),
-                ("done_s", ctypes.c_double), ("checks", ctypes.c_uint64), ("rollbacks", ctypes.c_uint64)]
+                ("done_s", ctypes.c_double), ("checks", ctypes.c_uint64), ("rollbacks", ctypes.c_uint64),
+                ("check_res64", ctypes.c_uint64)]
 
 
 def load(path: str = LIB_PATH) -> ctypes.CDLL:

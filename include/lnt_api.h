@@ -198,6 +198,7 @@ typedef struct lnt_result {
     int32_t status;          /* lnt_status of this test */
     double submit_s, admit_s, done_s;  /* seconds since lnt_server_create */
     uint64_t checks, rollbacks;        /* Gerbicz checks passed / rollbacks (PRP) */
+    uint64_t check_res64;              /* res64 of the Gerbicz product at the last verified check */
 } lnt_result;
 int lnt_server_create(const lnt_server_config *cfg, lnt_server **out);
 /* Queue a test: `iterations` squarings from the start residue (LL: 4, PRP: 3). */

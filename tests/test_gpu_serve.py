@@ -82,3 +82,37 @@ def test_budget_limits_admission():
     assert peak <= 2
     for i, p in enumerate(ps):
         assert got[i]["res64"] == alone(p, False, 400)[0]
+
+
+def test_served_mix_sample_matches_gmp():
+    """The served mix (cuda_amd.workloads.served_requests, seed 0): its GMP-pinned sample of
+    full LL tests (p in [50k, 500k], two of them at length 2^11) and PRP chunks (p in
+    [5M, 40M], their first 10,000 iterations) arrives with its Poisson spacing (gaps
+    scaled by 1/4, at most 0.25 s) into one server with Gerbicz checks on; every res64,
+    every LL verdict and every PRP chunk's Gerbicz checksum equals GMP's."""
+    from helpers import golden_cases
+    from cuda_amd.workloads import served_sample
+    fixtures = {(c["p"], c["mode"], c["iters"]): c for c in golden_cases() if c.get("served")}
+    reqs = served_sample(0)
+    assert all((r.p, "PRP" if r.prp else "LL", r.iterations) in fixtures for r in reqs)
+    assert any(cuda_amd.secured_log2_length(r.p) == 11 for r in reqs if not r.prp)
+    srv = Server(hbm_budget=16 << 30, step_ms=5.0, gerbicz_block=1000, gerbicz_check=2)
+    prev = reqs[0].arrival
+    for r in reqs:
+        time.sleep(min(0.25, (r.arrival - prev) / 4))
+        prev = r.arrival
+        say("submit", r.index, r.p, "PRP" if r.prp else "LL", r.iterations)
+        srv.submit(r.p, r.prp, r.iterations, r.index)
+    got = collect(srv, len(reqs), timeout_s=240)
+    srv.close()
+    for r in reqs:
+        res = got[r.index]
+        c = fixtures[(r.p, "PRP" if r.prp else "LL", r.iterations)]
+        assert res["status"] == 0, (r.p, res)
+        assert res["res64"] == int(c["res64"][str(r.iterations)], 16), (r.index, r.p)
+        if r.prp:
+            assert res["checks"] == r.iterations // 2000 and res["rollbacks"] == 0, (r.p, res)
+            assert res["check_res64"] == int(c["dres64"][str(r.iterations)], 16), (r.index, r.p)
+        else:
+            assert not res["is_zero"]  # none of the sampled exponents gives a Mersenne prime
+        say("served", r.index, r.p, f"{res['done_s'] - res['submit_s']:.2f} s")
