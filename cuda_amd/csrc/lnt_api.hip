@@ -14,41 +14,9 @@
 
 #include "../../include/lnt_api.h"
 #include "lnt_launch.hpp"
+#include "lnt_tables.hpp"
 
 using namespace lnt;
-
-// ------------------------------------------------------------------ table kernels
-
-__global__ void lnt_k_weights(u64 Bw, u64 Binv, u64 ninv, u32 p, u32 n, u64 *w, u64 *uw) {
-    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (i == 0) {
-        w[0] = 1;
-        uw[0] = ninv;
-        return;
-    }
-    u32 r = (u32)((u64)p * i) & (n - 1);
-    w[i] = gl_pow(Bw, (u64)(n - r));
-    uw[i] = gl_mul(gl_pow(Binv, (u64)(n - r)), ninv);
-}
-
-__global__ void lnt_k_line_tw(u64 om, u32 B1, u32 B2, u32 L, u64 *tw) {
-    u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= L) return;
-    tw[i] = lnt_line_tw(om, B1, B2, i);
-}
-
-__global__ void lnt_k_tw4(u64 om, u32 C, u32 B1, u32 B2, u32 n, u64 *tw) {
-    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    tw[i] = lnt_tw4(om, C, B1, B2, i);
-}
-
-__global__ void lnt_k_tw4_3(u64 om, u32 C, u32 BA, u32 BB, u32 BC, u32 n, u64 *tw) {
-    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    tw[i] = lnt_tw4_3(om, C, BA, BB, BC, i);
-}
 
 // ------------------------------------------------------------------ context
 
@@ -71,6 +39,10 @@ struct lnt_ctx {
     u64 *twc2 = nullptr, *itwc2 = nullptr;
     u64 *twr2 = nullptr, *itwr2 = nullptr;
     u64 *f = nullptr;  // transform of the second operand of a product (allocated on first use)
+    // GRAPH_ITERS iterations of the large path on x (each owing its "-2" when LL), captured
+    // once and replayed: consecutive launches from a graph start sooner than from the stream
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
     LntParams P{};
     std::string err;
     std::string tune_report;  // "name = ms" per candidate of the last lnt_tune
@@ -118,114 +90,6 @@ static void free_buffers(lnt_ctx *c) {
             (void)hipFree(*b);
             *b = nullptr;
         }
-}
-
-template <class T>
-static GView<T> view(T *p, u64 n) {
-    return GView<T>{p, n};
-}
-
-// Word counts of the tables of a plan (0 where the plan has no such table).
-struct TableSizes {
-    u32 n, Lr, L2, R, A;  // residue, row line, second-level row line, column line, second-level column line
-    bool large;
-};
-static TableSizes table_sizes(const Plan &pl) {
-    TableSizes t;
-    t.n = pl.n;
-    t.large = !pl.small;
-    t.Lr = pl.small ? pl.n : pl.C;
-    t.L2 = pl.row3m ? pl.rBB * pl.rBC : pl.row3 ? pl.row3 * pl.row3 : 0;
-    t.R = pl.small ? 0 : pl.R;
-    t.A = (!pl.small && pl.col3) ? pl.R / pl.col3 : 0;
-    return t;
-}
-
-// Device pointers of one set of tables (and the work buffer y).
-struct TablePtrs {
-    u64 *y = nullptr, *w = nullptr, *uw = nullptr, *tw4 = nullptr, *itw4 = nullptr;
-    u64 *twr = nullptr, *itwr = nullptr, *twr2 = nullptr, *itwr2 = nullptr;
-    u64 *twc = nullptr, *itwc = nullptr, *twc2 = nullptr, *itwc2 = nullptr;
-};
-
-// Launch the kernels that fill every table of plan pl (exponent p) on stream s.
-static void fill_tables(const Plan &pl, u32 p, const TablePtrs &t, hipStream_t s) {
-    const TableSizes z = table_sizes(pl);
-    const u32 n = pl.n;
-    const u64 Bw = gl_pow(7, (GL_P - 1) / 192 / n * 5);
-    const u64 ninv = gl_inv(n);
-    hipLaunchKernelGGL(lnt_k_weights, dim3((n + 255) / 256), dim3(256), 0, s, Bw, gl_inv(Bw), ninv, p, n, t.w, t.uw);
-    const u32 Lr = z.Lr;
-    u64 omr = gl_pow(554, (GL_P - 1) / Lr);
-    // row line twiddles: first phase B1 x B2 (3-phase rows: row3 x row3^2, row3m x rBB rBC)
-    const u32 rB1 = pl.row3m ? pl.row3m : pl.row3 ? pl.row3 : pl.row.B1;
-    const u32 rB2 = pl.row3m ? pl.rBB * pl.rBC : pl.row3 ? pl.row3 * pl.row3 : pl.row.B2;
-    hipLaunchKernelGGL(lnt_k_line_tw, dim3((Lr + 255) / 256), dim3(256), 0, s, omr, rB1, rB2, Lr, t.twr);
-    hipLaunchKernelGGL(lnt_k_line_tw, dim3((Lr + 255) / 256), dim3(256), 0, s, gl_inv(omr), rB1, rB2, Lr, t.itwr);
-    if (z.large) {
-        u64 omc = gl_pow(554, (GL_P - 1) / pl.R);
-        u64 omn = gl_pow(554, (GL_P - 1) / n);
-        if (pl.col3) {
-            // phase 1 (BA x A line) and phase 2 (BB x BC line) twiddles, 3-phase slot order
-            const u32 A = z.A;
-            hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, s, omc, pl.col3, A, pl.R, t.twc);
-            hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, s, gl_inv(omc), pl.col3, A, pl.R,
-                               t.itwc);
-            u64 oma = gl_pow(554, (GL_P - 1) / A);
-            hipLaunchKernelGGL(lnt_k_line_tw, dim3((A + 255) / 256), dim3(256), 0, s, oma, pl.cBB, pl.cBC, A, t.twc2);
-            hipLaunchKernelGGL(lnt_k_line_tw, dim3((A + 255) / 256), dim3(256), 0, s, gl_inv(oma), pl.cBB, pl.cBC, A,
-                               t.itwc2);
-            hipLaunchKernelGGL(lnt_k_tw4_3, dim3((n + 255) / 256), dim3(256), 0, s, omn, pl.C, pl.col3, pl.cBB, pl.cBC, n,
-                               t.tw4);
-            hipLaunchKernelGGL(lnt_k_tw4_3, dim3((n + 255) / 256), dim3(256), 0, s, gl_inv(omn), pl.C, pl.col3, pl.cBB,
-                               pl.cBC, n, t.itw4);
-        } else {
-            hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, s, omc, pl.col.B1, pl.col.B2, pl.R,
-                               t.twc);
-            hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, s, gl_inv(omc), pl.col.B1,
-                               pl.col.B2, pl.R, t.itwc);
-            hipLaunchKernelGGL(lnt_k_tw4, dim3((n + 255) / 256), dim3(256), 0, s, omn, pl.C, pl.col.B1, pl.col.B2, n,
-                               t.tw4);
-            hipLaunchKernelGGL(lnt_k_tw4, dim3((n + 255) / 256), dim3(256), 0, s, gl_inv(omn), pl.C, pl.col.B1,
-                               pl.col.B2, n, t.itw4);
-        }
-    }
-    // second-phase row twiddles: a line of L2 = B2 words split (row3 x row3 or rBB x rBC)
-    if (z.L2) {
-        const u32 s1 = pl.row3m ? pl.rBB : pl.row3, s2 = pl.row3m ? pl.rBC : pl.row3;
-        u64 om2 = gl_pow(554, (GL_P - 1) / z.L2);
-        hipLaunchKernelGGL(lnt_k_line_tw, dim3((z.L2 + 255) / 256), dim3(256), 0, s, om2, s1, s2, z.L2, t.twr2);
-        hipLaunchKernelGGL(lnt_k_line_tw, dim3((z.L2 + 255) / 256), dim3(256), 0, s, gl_inv(om2), s1, s2, z.L2,
-                           t.itwr2);
-    }
-}
-
-// Kernel parameters of plan pl over residue x and the tables t.
-static LntParams params_of(const Plan &pl, u32 p, u64 *x, const TablePtrs &t) {
-    const TableSizes z = table_sizes(pl);
-    const u32 n = pl.n;
-    LntParams P{};
-    P.x = view<u64>(x, n);
-    P.y = view<u64>(t.y, n);
-    P.w = view<const u64>(t.w, n);
-    P.uw = view<const u64>(t.uw, n);
-    P.tw4 = view<const u64>(t.tw4, z.large ? n : 0);
-    P.itw4 = view<const u64>(t.itw4, z.large ? n : 0);
-    P.twr = view<const u64>(t.twr, z.Lr);
-    P.itwr = view<const u64>(t.itwr, z.Lr);
-    P.twr2 = view<const u64>(t.twr2, z.L2);
-    P.itwr2 = view<const u64>(t.itwr2, z.L2);
-    P.twc = view<const u64>(t.twc, z.R);
-    P.itwc = view<const u64>(t.itwc, z.R);
-    P.twc2 = view<const u64>(t.twc2, z.A);
-    P.itwc2 = view<const u64>(t.itwc2, z.A);
-    P.p = p;
-    P.n = n;
-    P.log_n = pl.log_n;
-    P.C = pl.C;
-    P.R = pl.R;
-    P.levels = pl.levels;
-    return P;
 }
 
 // Allocate and fill every table of plan c->pl on c->stream.
@@ -302,6 +166,39 @@ static LntParams params_for(const lnt_ctx *c, u64 *x, u64 *y, const u64 *f) {
 }
 
 // k squarings (and -2 for LL) of the residue in buffer x; large path: three launches each.
+static void drop_graph(lnt_ctx *c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+}
+
+// Iterations per replay of the captured iteration graph.
+static constexpr uint32_t GRAPH_ITERS = 16;
+
+static bool launch_iteration(const Plan &pl, const LntParams &P, bool sub2, hipStream_t s) {
+    return lnt_launch_col_fwd(pl, P, sub2, s) && lnt_launch_row(pl, P, s) && lnt_launch_col_inv(pl, P, s);
+}
+
+// Capture GRAPH_ITERS iterations of the residue buffer c->x (carry with "-2" for LL).
+static int build_graph(lnt_ctx *c) {
+    drop_graph(c);
+    const LntParams P = params_for(c, c->x, c->y, nullptr);
+    const bool ll = !c->prp;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    bool ok = true;
+    for (uint32_t i = 0; i < GRAPH_ITERS && ok; i++) ok = launch_iteration(c->pl, P, ll, c->stream);
+    hipError_t ec = hipStreamEndCapture(c->stream, &c->graph);
+    if (!ok || ec != hipSuccess) {
+        drop_graph(c);
+        return fail(c, ok ? LNT_DEVICE_ERROR : LNT_UNSUPPORTED, "iteration graph capture failed");
+    }
+    HIPCHK(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    return LNT_OK;
+}
+
+// k squarings (and -2 for LL) of the residue in buffer x; large path: three launches each
+// (on the context's own residue, whole groups of GRAPH_ITERS run as one graph launch).
 static int square_buf(lnt_ctx *c, u64 *x, bool &pending, uint32_t k) {
     const Plan &pl = c->pl;
     const bool ll = !c->prp;
@@ -317,9 +214,24 @@ static int square_buf(lnt_ctx *c, u64 *x, bool &pending, uint32_t k) {
             k -= s;
         }
     } else {
+        // the first iteration owes no "-2" when the residue was just set (or multiplied)
+        if (k && ll && !pending) {
+            if (!launch_iteration(pl, P, false, c->stream))
+                return fail(c, LNT_UNSUPPORTED, "no kernel for this row/column split");
+            pending = true;
+            k--;
+        }
+        if (x == c->x && k >= GRAPH_ITERS) {
+            if (!c->graph_exec) {
+                int rc = build_graph(c);
+                if (rc != LNT_OK) return rc;
+            }
+            for (; k >= GRAPH_ITERS; k -= GRAPH_ITERS)
+                HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch");
+            pending = true;
+        }
         for (uint32_t i = 0; i < k; i++) {
-            if (!lnt_launch_col_fwd(pl, P, ll && pending, c->stream) || !lnt_launch_row(pl, P, c->stream) ||
-                !lnt_launch_col_inv(pl, P, c->stream))
+            if (!launch_iteration(pl, P, ll && pending, c->stream))
                 return fail(c, LNT_UNSUPPORTED, "no kernel for this row/column split");
             pending = true;
         }
@@ -541,6 +453,7 @@ void lnt_destroy(lnt_ctx *c) {
     if (c->batch) (void)lnt_batch_remove(c->batch, c);
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
     free_buffers(c);
     if (c->ev_mark) (void)hipEventDestroy(c->ev_mark);
     if (c->ev_gz) (void)hipEventDestroy(c->ev_gz);
@@ -876,6 +789,7 @@ int lnt_tune(lnt_ctx *c, int loops) {
     std::swap(c->twc2, nc->twc2);
     std::swap(c->itwc2, nc->itwc2);
     std::swap(c->P, nc->P);
+    drop_graph(c);  // captured over the old buffers and kernels
     lnt_destroy(nc);
     return LNT_OK;
 }

@@ -247,3 +247,30 @@ def test_device_field_ops_equal_host_forms():
     forms, which the oracle tests pin to Python integers mod P."""
     from cuda_amd.lnt import field_selftest
     assert field_selftest(1 << 18, seed=7) == 0
+
+
+@pytest.mark.parametrize("prp", [False, True])
+def test_graph_replays_match_oracle(prp):
+    """Long iteration runs replay a captured graph of 16 iterations: a run of 37 from a set
+    residue (one eager iteration without the LL "-2", two replays, four eager), then 16
+    more (one replay), against the oracle; the graph is rebuilt after the tuner changes
+    the context."""
+    import numpy as np
+    from helpers import random_digits_np
+    p, n = 2000003, 1 << 17
+    d = random_digits_np(p, n, seed=77 + prp)
+    o = OracleTest(p, n, prp, n0=256)
+    o.set_digits_np(d)
+    t = cuda_amd.LucasPRP(p, n, prp=prp)
+    t.set_digits(d.tolist())
+    for k in (37, 16):
+        o.iterate(k)
+        t.iterate(k)
+        got = np.asarray(t.get_digits_buffer(), dtype=np.uint64)
+        assert np.array_equal(got, o.digits_np()), (prp, k)
+    t.tune(20)
+    o.iterate(40)
+    t.iterate(40)
+    got = np.asarray(t.get_digits_buffer(), dtype=np.uint64)
+    assert np.array_equal(got, o.digits_np()), (prp, "after tune")
+    t.close()
