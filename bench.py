@@ -5,9 +5,19 @@ Workload (the reference's single-test throughput configuration): LL test of M_p 
 p = smallest prime >= 70,000,000 + (seed mod 10^5) (p = 70,000,027 for seed 0), NTT length
 2^22 u64 words as the reference's length selection assigns (~16.7 bits/word), start
 residue 4.  A step is one LL iteration (one squaring and -2) over the whole residue, which
-stays resident in HBM.  Prints one JSON line.
+stays resident in HBM.  With the default warm-up and step counts the run ends on
+iteration 3,000, whose res64 is checked against the GMP fixture.  Prints one JSON line.
 
-  python bench.py [--steps K] [--warmup W] [--seed S] [--cpu-seconds T]
+Further legs in the same line (each can be switched off):
+  batched  64 LL tests of p log-uniform in [1M, 20M], 10,000 iterations each, as one batch
+           and one at a time on the plain path;
+  served   the first requests of the served mix (Poisson 4/s) through the serving loop at a
+           reduced iteration scale, against the same stream one at a time;
+  checked  the PRP test of p = 140,000,041 at 2^23 with and without the Gerbicz check;
+  cpu_baseline  the oracle's C restatement on the host cores, a bounded sample.
+
+  python bench.py [--steps K] [--warmup W] [--seed S] [--no-cpu] [--batch-iters 0]
+                  [--served-requests 0] [--no-checked]
 """
 import argparse
 import json
@@ -21,8 +31,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms/iter @2^22-word NTT (+% HBM roofline)"
 HBM_PEAK_GBS = 8000.0
-# integer VALU: 256 CUs x 64 lanes x 2.4 GHz (one 32-bit lane-op per lane per clock)
-VALU_PEAK = 256 * 64 * 2.4e9
+LANES_PER_CU_CLOCK = 64  # one 32-bit VALU instruction per lane per clock and CU (4 SIMD x 16)
 
 
 def is_prime(x: int) -> bool:
@@ -48,16 +57,28 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(p: int, n: int, seconds: float):
+def golden_res64(p: int, mode: str):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    try:
+        from helpers import golden  # the GMP fixtures
+        return golden(p, mode) or {}
+    except Exception as e:  # fixtures are optional for the measurement
+        log("bench: no fixture check:", e)
+        return {}
+
+
+def cpu_baseline(p: int, n: int, seconds: float, gpu_check):
     """The oracle's C restatement of the same weighted NTT squaring (OpenMP over rows and
-    columns), timed on this host for a bounded sample of iterations."""
+    columns), timed on this host for a bounded sample of iterations after 10 warm-up
+    iterations; its res64 is compared with the GPU's after the same number of iterations."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from helpers import OracleTest, oracle  # test infrastructure: the CPU baseline leg
     L = oracle()
     o = OracleTest(p, n, False, n0=1 << (int(math.log2(n)) // 2))
+    warm = 10
     t0 = time.perf_counter()
-    o.iterate(2)
-    per = (time.perf_counter() - t0) / 2
+    o.iterate(warm)
+    per = (time.perf_counter() - t0) / warm
     k = max(2, int(seconds / max(per, 1e-6)))
     t0 = time.perf_counter()
     o.iterate(k)
@@ -70,10 +91,13 @@ def cpu_baseline(p: int, n: int, seconds: float):
                 break
     except OSError:
         pass
+    r_cpu = o.res64()
+    r_gpu = gpu_check(warm + k)
     return {"value": round(dt * 1000.0 / k, 3), "unit": "ms/iter", "cores": int(L.lo_threads()),
             "kind": "port",
-            "sample": f"{k} LL iterations of M{p} at n=2^{int(math.log2(n))} after 2 warm-up "
-                      f"iterations ({dt:.1f} s), oracle/lnt_oracle.c, CPU: {cpu}"}
+            "sample": f"{k} LL iterations of M{p} at n=2^{int(math.log2(n))} after {warm} warm-up "
+                      f"iterations ({dt:.1f} s), oracle/lnt_oracle.c, CPU: {cpu}",
+            "res64": f"{r_cpu:016X}", "res64_equal_to_gpu": r_cpu == r_gpu}
 
 
 def batched_leg(seed: int, iters: int, chunk: int = 1000):
@@ -111,18 +135,13 @@ def batched_leg(seed: int, iters: int, chunk: int = 1000):
     tp = time.perf_counter() - t0
     log(f"bench: one at a time {tp:.2f} s")
     gmp_checked = gmp_equal = 0
-    try:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from helpers import golden
-        for p, r in zip(ps, res_b):
-            g = golden(p, "LL") or {}
-            if iters in g:
-                gmp_checked += 1
-                gmp_equal += int(g[iters] == r)
-    except Exception as e:  # fixtures are optional for the measurement
-        log("bench: no fixture check:", e)
+    for p, r in zip(ps, res_b):
+        g = golden_res64(p, "LL")
+        if iters in g:
+            gmp_checked += 1
+            gmp_equal += int(g[iters] == r)
     return {"workload": f"{len(ps)} LL tests, p log-uniform in [1e6, 2e7] (splitmix64 seed {seed}), "
-                        f"n = 2^{min(t for t in map(secured, ps))}..2^{max(map(secured, ps))}, "
+                        f"n = 2^{min(map(secured, ps))}..2^{max(map(secured, ps))}, "
                         f"{iters} iterations each",
             "tests": len(ps), "iterations": iters,
             "batched_s": round(tb, 3), "plain_s": round(tp, 3),
@@ -133,16 +152,88 @@ def batched_leg(seed: int, iters: int, chunk: int = 1000):
             "res64_checked_against_gmp": gmp_checked, "res64_equal_to_gmp": gmp_equal}
 
 
+def checked_leg(p: int = 140_000_041, block: int = 1000, iters: int = 4000):
+    """PRP at 2^23 with the Gerbicz check: ms/iter unchecked, with the block products only
+    (blocks per check larger than the run), and with one check every 4 blocks.  The cost of
+    a block product and of a check give the overhead at any check interval L; the smallest
+    L with overhead <= 2% is reported.  res64 and the checksum at `iters` are compared with
+    the GMP fixture."""
+    from cuda_amd import LucasPRP
+
+    def run(mode):
+        t = LucasPRP(p, prp=True)  # starts from 3
+        if mode != "plain":  # set-up of the checkpoint slots, outside the timed region
+            t.gerbicz_begin(block, 10 ** 6 if mode == "blocks" else 4)
+        t.sync()
+        t0 = time.perf_counter()
+        if mode == "plain":
+            t.iterate(iters)
+            t.sync()
+            st = None
+        else:
+            t.gerbicz_iterate(iters)
+            t.sync()
+            st = t.gerbicz_stats()
+        dt = time.perf_counter() - t0
+        r = t.res64()
+        t.close()
+        return dt, r, st
+    t_plain, r_plain, _ = run("plain")
+    t_blocks, r_blocks, _ = run("blocks")
+    t_check, r_check, st = run("check")
+    nb = iters // block
+    t_iter = t_plain / iters
+    block_cost = max(0.0, (t_blocks - t_plain) / nb)
+    check_cost = max(0.0, t_check - t_blocks)
+    L = 1
+    while (L * block_cost + check_cost) / (L * block * t_iter) > 0.02 and L < 10 ** 4:
+        L += 1
+    g = golden_res64(p, "PRP")
+    dg = {}
+    try:
+        from helpers import golden_cases
+        for c in golden_cases():
+            if c["p"] == p and "dres64" in c:
+                dg.update({int(k): int(v, 16) for k, v in c["dres64"].items()})
+    except Exception:
+        pass
+    return {"workload": f"PRP test of M{p} (n=2^23), {iters} iterations, Gerbicz block {block}",
+            "ms_per_iter_unchecked": round(t_plain * 1e3 / iters, 5),
+            "ms_per_iter_block_products": round(t_blocks * 1e3 / iters, 5),
+            "ms_per_iter_check_every_4_blocks": round(t_check * 1e3 / iters, 5),
+            "block_product_ms": round(block_cost * 1e3, 3), "check_ms": round(check_cost * 1e3, 3),
+            "overhead_check_every_4_blocks": round(t_check / t_plain - 1, 4),
+            "blocks_per_check_for_2pct": L,
+            "overhead_at_that_interval": round((L * block_cost + check_cost) / (L * block * t_iter), 4),
+            "checks_passed": st["checks"] if st else None,
+            "res64_equal": r_plain == r_blocks == r_check,
+            "res64_matches_gmp": (g[iters] == r_check) if iters in g else None,
+            "checksum_matches_gmp": (dg[iters] == st["checkpoint_res64"]) if (st and iters in dg) else None}
+
+
 def secured(p: int) -> int:
     from cuda_amd import secured_log2_length
     return secured_log2_length(p)
+
+
+def counters_for(n: int, name: str):
+    """HBM bytes and VALU lane-instructions per launch of kernel `name` at length n, from the
+    rocprofv3 counter runs summarised in profiles/ (exact kernel name and length only)."""
+    path = os.path.join(ROOT, "profiles", "traffic_2p22.json")
+    try:
+        doc = json.load(open(path))
+    except (OSError, ValueError):
+        return {}
+    if doc.get("n") != n:
+        return {}
+    return doc.get("kernels", {}).get(name, {})
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--profile-iters", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -152,6 +243,10 @@ def main() -> int:
     ap.add_argument("--prp", action="store_true", help="PRP (start 3, no -2) instead of LL")
     ap.add_argument("--batch-iters", type=int, default=10000,
                     help="iterations per test of the batched workload (0: skip that leg)")
+    ap.add_argument("--served-requests", type=int, default=120,
+                    help="requests of the served mix for the served leg (0: skip it)")
+    ap.add_argument("--served-scale", type=float, default=0.1, help="iteration scale of the served leg")
+    ap.add_argument("--no-checked", action="store_true", help="skip the Gerbicz-checked PRP leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain copy-like kernel (int64 negate) over MIB MiB, three "
                          "times: the reference point that scales HBM byte counters")
@@ -162,8 +257,8 @@ def main() -> int:
               f"is not run.", file=sys.stderr)
         return 2
 
-    import cuda_amd
     from cuda_amd import LucasPRP
+    from cuda_amd.lnt import valu_rates
 
     if args.copy_calibration:
         import torch
@@ -178,9 +273,11 @@ def main() -> int:
         del a, b
 
     p = args.exponent or next_prime(70_000_000 + args.seed % 100_000)
+    mode = "PRP" if args.prp else "LL"
     t = LucasPRP(p, prp=args.prp)
     n = t.n
-    log(f"bench: p={p} n={n} plan={t.describe()}")
+    plan = t.describe()
+    log(f"bench: p={p} n={n} plan={plan}")
 
     t.iterate(args.warmup)
     t.sync()
@@ -192,38 +289,28 @@ def main() -> int:
     ms_per_step = wall * 1000.0 / args.steps
     gpu_ms_per_step = gpu_ms / args.steps
     done = args.warmup + args.steps
+    res64 = t.res64()
+    g = golden_res64(p, mode)
+    check = (g[done] == res64) if done in g else None
 
-    # parity spot check against the GMP fixture, when this iteration count has one
-    check = None
-    try:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from helpers import golden
-        g = golden(p, "PRP" if args.prp else "LL") or {}
-        if done in g:
-            check = t.res64() == g[done]
-    except Exception as e:  # fixtures are optional for the measurement
-        log("bench: no fixture check:", e)
-
+    # per-kernel durations: each kernel `profile_iters` times between two HIP events on the
+    # test's stream (the residue is saved and restored around it)
     prof = t.profile(args.profile_iters)
+    t.close()
     name, kms = max(prof, key=lambda kv: kv[1])
     alg_bytes = 16 * n  # one read and one write of the n-word residue per launch
     achieved = alg_bytes / (kms * 1e-3) / 1e9
-    iter_ms = sum(v for _, v in prof)
-    # HBM bytes and VALU lane-ops per launch of each kernel, from the rocprofv3 counter runs
-    # summarised in profiles/ (FETCH_SIZE / WRITE_SIZE scaled by the copy calibration)
-    counters = {}
-    tfile = os.path.join(ROOT, "profiles", "traffic_2p22.json")
-    if n == 1 << 22 and os.path.exists(tfile):
-        try:
-            counters = json.load(open(tfile)).get("kernels", {}).get(name.split("<")[0], {})
-        except Exception:
-            counters = {}
-    traffic = counters.get("traffic_bytes")
+    iter_bytes = 32 * n  # two read+write sweeps of the residue per iteration
+    rates = valu_rates()
+    ghz = rates.pop("ghz")
+    counters = counters_for(n, name)
     valu = None
     if counters.get("valu_lane_ops"):
+        peak = 256 * LANES_PER_CU_CLOCK * ghz * 1e9
         va = counters["valu_lane_ops"] / (kms * 1e-3)
-        valu = {"achieved": round(va / 1e12, 2), "peak": VALU_PEAK / 1e12, "unit": "T lane-ops/s",
-                "frac": round(va / VALU_PEAK, 4), "lane_ops_per_launch": counters["valu_lane_ops"]}
+        valu = {"achieved": round(va / 1e12, 2), "peak": round(peak / 1e12, 2), "unit": "T lane-instr/s",
+                "frac": round(va / peak, 4), "lane_instr_per_launch": counters["valu_lane_ops"],
+                "clock_ghz_measured": round(ghz, 3)}
     line = {
         "metric": METRIC,
         "value": round(ms_per_step, 5),
@@ -237,24 +324,36 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": f"{'PRP' if args.prp else 'LL'} test of M{p}, n=2^{n.bit_length() - 1} u64 "
-                               f"words (GF(2^64-2^32+1) IBDWT), iterations {args.warmup + 1}..{done}",
-                   "p": p, "n": n, "plan": t.describe()},
+        "config": {"workload": f"{mode} test of M{p}, n=2^{n.bit_length() - 1} u64 words "
+                               f"(GF(2^64-2^32+1) IBDWT), iterations {args.warmup + 1}..{done}",
+                   "p": p, "n": n, "plan": plan},
         "gpu_ms_per_step": round(gpu_ms_per_step, 5),
-        "res64": f"{t.res64():016X}",
+        "res64": f"{res64:016X}",
         "res64_matches_gmp": check,
         "kernels_ms": {k: round(v, 5) for k, v in prof},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "valu": valu,
-                     "iteration": {"bytes": 32 * n, "ms": round(iter_ms, 5),
-                                   "achieved": round(32 * n / (iter_ms * 1e-3) / 1e9, 1),
-                                   "frac": round(32 * n / (iter_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
+                     "traffic": counters.get("traffic_bytes"), "valu": valu,
+                     "iteration": {"bytes": iter_bytes, "ms": round(ms_per_step, 5),
+                                   "achieved": round(iter_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+                                   "frac": round(iter_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
+        "valu_field_ops_per_cu_clock": {k: round(v, 3) for k, v in rates.items()},
     }
     if args.batch_iters:
         line["batched"] = batched_leg(args.seed, args.batch_iters)
+    if args.served_requests:
+        from cuda_amd.serve import served_mix
+        line["served"] = served_mix(requests=args.served_requests, iter_scale=args.served_scale, verify=6)
+    if not args.no_checked:
+        line["checked"] = checked_leg()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(p, n, args.cpu_seconds)
+        def gpu_res64_after(k):
+            u = LucasPRP(p, prp=args.prp)
+            u.iterate(k)
+            r = u.res64()
+            u.close()
+            return r
+        line["cpu_baseline"] = cpu_baseline(p, n, args.cpu_seconds, gpu_res64_after)
     print(json.dumps(line), flush=True)
     return 0
 

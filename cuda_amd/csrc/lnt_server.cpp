@@ -25,6 +25,11 @@
 
 #include "../../include/lnt_api.h"
 
+// Default Gerbicz check interval in blocks.  A check costs one block of squarings plus
+// reading both checked residues back (measured at 2^23, block 1000: 336 ms against
+// 246 ms per 1000 iterations), so the overhead is about 1.4 / L: 50 blocks keep it near 3%.
+constexpr uint32_t kDefaultCheckBlocks = 50;
+
 namespace {
 
 struct Pending {
@@ -148,7 +153,16 @@ struct lnt_server {
             a.cls = size_class(q.p);
             a.admit_s = now();
             int rc = lnt_create(cfg.device, q.p, nullptr, 0, q.prp ? LNT_MODE_PRP : LNT_MODE_LL, &a.ctx);
-            if (rc == LNT_OK && a.gz) rc = lnt_gerbicz_begin(a.ctx, cfg.gerbicz_block, cfg.gerbicz_check);
+            if (rc == LNT_OK && a.gz) {
+                // blocks per check: the configured value, or by default one check every
+                // kDefaultCheckBlocks blocks and never fewer than one per request
+                uint32_t L = cfg.gerbicz_check;
+                if (!L) {
+                    const uint64_t blocks = q.iterations / cfg.gerbicz_block;
+                    L = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kDefaultCheckBlocks, blocks));
+                }
+                rc = lnt_gerbicz_begin(a.ctx, cfg.gerbicz_block, L);
+            }
             if (rc == LNT_OK) rc = lnt_batch_add(batch[a.cls], a.ctx);
             admitted++;
             if (rc != LNT_OK || q.iterations == 0) {
@@ -225,7 +239,7 @@ int lnt_server_create(const lnt_server_config *cfg, lnt_server **out) {
     lnt_server *s = new lnt_server;
     s->cfg = *cfg;
     if (s->cfg.step_ms <= 0) s->cfg.step_ms = 5.0f;
-    if (s->cfg.gerbicz_block && !s->cfg.gerbicz_check) s->cfg.gerbicz_check = 10;
+
     for (int c = 0; c < CLASSES; c++) {
         int rc = lnt_batch_create(cfg->device, &s->batch[c]);
         if (rc != LNT_OK) {

@@ -36,7 +36,7 @@ EXPORTS = (
     "lnt_batch_destroy", "lnt_batch_last_error", "lnt_server_create", "lnt_server_submit",
     "lnt_server_poll", "lnt_server_stats", "lnt_server_now", "lnt_server_destroy",
     "lnt_server_last_error", "lnt_selftest", "lnt_tune_export", "lnt_tune_import",
-    "lnt_tune_results", "lnt_iterate_static",
+    "lnt_tune_results", "lnt_iterate_static", "lnt_valu_rates",
 )
 
 
@@ -114,6 +114,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lnt_batch_last_error.argtypes = [vp]
     lib.lnt_batch_last_error.restype = ctypes.c_char_p
     lib.lnt_selftest.argtypes = [ctypes.c_int, u32, ctypes.c_uint64, u64p]
+    lib.lnt_valu_rates.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_char_p,
+                                   ctypes.c_size_t, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
     lib.lnt_tune_export.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
     lib.lnt_tune_import.argtypes = [ctypes.c_char_p]
     lib.lnt_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
@@ -156,6 +158,21 @@ def field_selftest(count: int = 1 << 16, seed: int = 1, device: int = 0) -> int:
     if rc != LNT_OK:
         raise LntError(rc, lib.lnt_last_error(None).decode())
     return int(bad.value)
+
+
+def valu_rates(device: int = 0) -> dict:
+    """Field operations per CU per measured clock on the device ({name: rate}, plus "ghz")."""
+    lib = load()
+    cap = 16
+    rates = (ctypes.c_double * cap)()
+    names = ctypes.create_string_buffer(512)
+    ghz = ctypes.c_double()
+    k = lib.lnt_valu_rates(device, rates, names, len(names), ctypes.byref(ghz), cap)
+    if k < 0:
+        raise LntError(-k, "lnt_valu_rates failed")
+    out = dict(zip(names.value.decode().split(";"), [float(rates[i]) for i in range(k)]))
+    out["ghz"] = float(ghz.value)
+    return out
 
 
 def secured_log2_length(p: int) -> int:
@@ -321,7 +338,6 @@ class LucasPRP:
             raise LntError(-cnt, self._lib.lnt_last_error(self._h).decode())
         if cnt == 0:
             raise LntError(LNT_DEVICE_ERROR, self._lib.lnt_last_error(self._h).decode())
-        self.iteration += int(iters)
         return list(zip(names.value.decode().split(";"), [float(ms[i]) for i in range(cnt)]))
 
 

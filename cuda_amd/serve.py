@@ -132,32 +132,24 @@ def run_alone(r: Request, s: float, block: int, check: int):
     return v, dt
 
 
-def main(argv=None) -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--requests", type=int, default=2000)
-    ap.add_argument("--rate", type=float, default=4.0)
-    ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--iter-scale", type=float, default=1.0)
-    ap.add_argument("--budget-gb", type=float, default=200.0)
-    ap.add_argument("--step-ms", type=float, default=5.0)
-    ap.add_argument("--gerbicz-block", type=int, default=1000)
-    ap.add_argument("--gerbicz-check", type=int, default=0, help="blocks per check (0: up to 10 per chunk)")
-    ap.add_argument("--verify", type=int, default=24, help="requests also run alone for real")
-    ap.add_argument("--json", default="")
-    a = ap.parse_args(argv)
-    reqs = served_requests(a.seed, a.requests, a.rate)
-    prp_iters = max(scaled(r, a.iter_scale) for r in reqs if r.prp) if any(r.prp for r in reqs) else 0
-    block = a.gerbicz_block if prp_iters >= a.gerbicz_block else 0
-    check = a.gerbicz_check or max(1, min(10, prp_iters // max(1, a.gerbicz_block)))
-    log(f"serve: {len(reqs)} requests over {reqs[-1].arrival:.0f} s, iteration scale {a.iter_scale}, "
+def served_mix(requests: int = 2000, rate: float = 4.0, seed: int = 0, iter_scale: float = 1.0,
+               budget_gb: float = 200.0, step_ms: float = 5.0, gerbicz_block: int = 1000,
+               gerbicz_check: int = 0, verify: int = 24) -> dict:
+    """Serve the first `requests` of the mix at their arrival times and model the same stream
+    one at a time on the plain path; returns the report (tests/hour, p50/p99 both ways)."""
+    reqs = served_requests(seed, requests, rate)
+    prp_iters = max(scaled(r, iter_scale) for r in reqs if r.prp) if any(r.prp for r in reqs) else 0
+    block = gerbicz_block if prp_iters >= gerbicz_block else 0
+    check = gerbicz_check or max(1, min(10, prp_iters // max(1, gerbicz_block)))
+    log(f"serve: {len(reqs)} requests over {reqs[-1].arrival:.0f} s, iteration scale {iter_scale}, "
         f"Gerbicz block {block} x {check}")
 
-    classes = plain_classes(reqs, a.iter_scale, block, check)
-    service = [plain_service(r, a.iter_scale, classes) for r in reqs]
+    classes = plain_classes(reqs, iter_scale, block, check)
+    service = [plain_service(r, iter_scale, classes) for r in reqs]
     pdone = fifo(reqs, service)
     p_ttr = [d - r.arrival for d, r in zip(pdone, reqs)]
 
-    got = run_served(reqs, a.iter_scale, int(a.budget_gb * 2**30), a.step_ms, block, check)
+    got = run_served(reqs, iter_scale, int(budget_gb * 2**30), step_ms, block, check)
     meta = got.pop(-1)
     ttr = [got[r.index]["done_s"] - got[r.index]["submit_s"] for r in reqs]
     errors = sum(1 for v in got.values() if v["status"] != 0)
@@ -165,20 +157,21 @@ def main(argv=None) -> int:
     p_makespan = pdone[-1] - reqs[0].arrival
 
     # a sample run alone on the plain path: same res64, and a check of the service model
-    sample = sorted(set(list(range(min(a.verify // 2, len(reqs))))
-                        + [int(i * len(reqs) / max(1, a.verify // 2)) for i in range(a.verify // 2)]))
+    sample = sorted(set(list(range(min(verify // 2, len(reqs))))
+                        + [int(i * len(reqs) / max(1, verify // 2)) for i in range(verify // 2)]))
     equal, model_err = 0, []
     for i in sample:
         r = reqs[i]
-        v, dt = run_alone(r, a.iter_scale, block, check)
+        v, dt = run_alone(r, iter_scale, block, check)
         equal += int(v == got[r.index]["res64"])
         model_err.append((service[i] - dt) / dt)
-    out = {
-        "workload": f"{len(reqs)} requests, Poisson rate {a.rate}/s (seed {a.seed}): 80% full LL "
+    return {
+        "workload": f"{len(reqs)} requests, Poisson rate {rate}/s (seed {seed}): 80% full LL "
                     f"p in [50k, 500k], 20% PRP chunks of 100,000 iterations p in [5M, 40M]; "
-                    f"iterations x {a.iter_scale}; Gerbicz block {block} x {check}; HBM budget "
-                    f"{a.budget_gb:g} GB",
-        "requests": len(reqs), "rate": a.rate, "iter_scale": a.iter_scale,
+                    f"iterations x {iter_scale}; Gerbicz block {block} x {check}; HBM budget "
+                    f"{budget_gb:g} GB",
+        "requests": len(reqs), "rate": rate, "iter_scale": iter_scale,
+        "arrival_window_s": round(reqs[-1].arrival - reqs[0].arrival, 2),
         "served": {"tests_per_hour": round(len(reqs) / makespan * 3600, 1),
                    "makespan_s": round(makespan, 2),
                    "p50_s": round(percentile(ttr, 50), 3), "p99_s": round(percentile(ttr, 99), 3),
@@ -197,12 +190,30 @@ def main(argv=None) -> int:
                            "service_model_error_median": round(sorted(model_err)[len(model_err) // 2], 3)
                            if model_err else None},
     }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--iter-scale", type=float, default=1.0)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--step-ms", type=float, default=5.0)
+    ap.add_argument("--gerbicz-block", type=int, default=1000)
+    ap.add_argument("--gerbicz-check", type=int, default=0, help="blocks per check (0: up to 10 per chunk)")
+    ap.add_argument("--verify", type=int, default=24, help="requests also run alone for real")
+    ap.add_argument("--json", default="")
+    a = ap.parse_args(argv)
+    out = served_mix(a.requests, a.rate, a.seed, a.iter_scale, a.budget_gb, a.step_ms, a.gerbicz_block,
+                     a.gerbicz_check, a.verify)
     line = json.dumps(out)
     print(line, flush=True)
     if a.json:
         with open(a.json, "w") as f:
             f.write(line + "\n")
-    return 0 if errors == 0 and equal == len(sample) else 1
+    ok = out["served"]["errors"] == 0 and out["verified_alone"]["res64_equal"] == out["verified_alone"]["count"]
+    return 0 if ok else 1
 
 
 if __name__ == "__main__":

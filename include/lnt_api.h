@@ -103,6 +103,13 @@ int lnt_tune_import(const char *text);
  * draws from `seed`) and compared with the host forms; *mismatches = differing results. */
 int lnt_selftest(int device, uint32_t count, uint64_t seed, uint64_t *mismatches);
 
+/* VALU throughput of the device field arithmetic (new: a measurement aid): each of
+ * gl_mul, gl_sqr, gl_add, gl_sub, the three shift-multiply forms and one v_mad_u64_u32 runs
+ * as eight independent chains per thread over a grid filling every CU.  ops_per_clk[i] =
+ * operations per CU per measured shader clock; names gets "name;name;..."; *ghz the mean
+ * measured clock.  Returns the count (up to cap), or a negative lnt_status. */
+int lnt_valu_rates(int device, double *ops_per_clk, char *names, size_t names_len, double *ghz, int cap);
+
 /* Number of digits n of a context, and its description: "name;kernel0;kernel1;..." used
  * for the stats file's Pool / Kernel<i> keys (lucas_prp.cu:311-313). */
 uint32_t lnt_length(const lnt_ctx *ctx);
@@ -186,7 +193,7 @@ typedef struct lnt_server_config {
     uint32_t max_tests;      /* admitted tests at most (0: no limit) */
     float step_ms;           /* target GPU time of one scheduling step (0: 5 ms) */
     uint32_t gerbicz_block;  /* PRP tests: Gerbicz block (0: unchecked) */
-    uint32_t gerbicz_check;  /* blocks per check (0: 10) */
+    uint32_t gerbicz_check;  /* blocks per check (0: 50, or the request's block count if fewer) */
 } lnt_server_config;
 typedef struct lnt_result {
     uint64_t tag;            /* the caller's id given to lnt_server_submit */
