@@ -293,8 +293,8 @@ def main() -> int:
     g = golden_res64(p, mode)
     check = (g[done] == res64) if done in g else None
 
-    # per-kernel durations: each kernel `profile_iters` times between two HIP events on the
-    # test's stream (the residue is saved and restored around it)
+    # per-kernel durations: each kernel `profile_iters` times back to back between two HIP
+    # events on the test's stream (the residue is saved and restored around it)
     prof = t.profile(args.profile_iters)
     t.close()
     name, kms = max(prof, key=lambda kv: kv[1])

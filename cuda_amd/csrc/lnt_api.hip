@@ -618,8 +618,10 @@ int lnt_profile(lnt_ctx *c, uint32_t iters, char *names, size_t names_len, float
     const Plan &pl = c->pl;
     const bool ll = !c->prp;
     const int nk = pl.small ? 1 : 3;
-    // Each kernel runs `iters` times back to back between two events (the residue is
-    // saved first and restored after: repeated launches of one kernel are timing only).
+    // Each kernel runs `iters` times back to back between two events on the context's
+    // stream (the residue is saved first and restored after: repeated launches of one
+    // kernel are timing only).  Events between the kernels of every iteration would add
+    // their own gaps to each duration; back-to-back launches agree with the kernel trace.
     u64 *save = nullptr;
     HIPCHK(c, hipMalloc(&save, (size_t)pl.n * 8), "hipMalloc (profile)");
     HIPCHK(c, hipMemcpyAsync(save, c->x, (size_t)pl.n * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpy");
@@ -965,11 +967,15 @@ struct lnt_batch {
     hipStream_t stream = nullptr;
     hipEvent_t ev_mark = nullptr;
     std::vector<lnt_ctx *> members;
+    // Members of one kernel shape share one batched launch per kernel over `cap` parameter
+    // slots; an empty slot (n = 0) launches workgroups that return at once, so members can
+    // join or leave a group with free slots by rewriting its parameters, without a new graph.
     struct Group {
         Plan pl;
         bool ll = false;
-        std::vector<lnt_ctx *> ctxs;
-        std::vector<LntParams> h;
+        u32 cap = 0;
+        std::vector<lnt_ctx *> ctxs;  // cap slots, nullptr = empty
+        std::vector<LntParams> h;     // cap slots, n = 0 = empty
         LntParams *d = nullptr;
         hipStream_t side = nullptr;
         hipEvent_t join = nullptr;
@@ -979,7 +985,9 @@ struct lnt_batch {
     hipEvent_t done = nullptr;  // end of the last batch_iterate: members' streams wait on it
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    bool dirty = true;
+    bool dirty = true;      // the groups or their capacities changed: capture a new graph
+    bool upload = false;    // parameters changed: copy them to the device before the next step
+    uint64_t captures = 0;  // graphs captured so far
     std::string err;
 };
 
@@ -1013,20 +1021,33 @@ static void batch_release(lnt_batch *b) {
     b->groups.clear();
 }
 
-// Launch one iteration of group g on stream s.
+// Launch one iteration of group g on stream s (all cap slots; empty ones return at once).
 static bool batch_group_step(lnt_batch::Group &g, hipStream_t s) {
-    const u32 cnt = (u32)g.ctxs.size();
-    if (g.pl.small) return lnt_launch_batch_small(g.pl, g.d, cnt, 1, g.ll, s);
-    return lnt_launch_batch_col_fwd(g.pl, g.d, cnt, s) && lnt_launch_batch_row(g.pl, g.d, cnt, s) &&
-           lnt_launch_batch_col_inv(g.pl, g.d, cnt, s);
+    if (g.pl.small) return lnt_launch_batch_small(g.pl, g.d, g.cap, 1, g.ll, s);
+    return lnt_launch_batch_col_fwd(g.pl, g.d, g.cap, s) && lnt_launch_batch_row(g.pl, g.d, g.cap, s) &&
+           lnt_launch_batch_col_inv(g.pl, g.d, g.cap, s);
 }
 
+// The parameter block of member c in a batch launch (its "-2" flag from its own state).
+static LntParams member_params(const lnt_ctx *c) {
+    LntParams P = c->P;
+    P.sub2 = (!c->prp && c->pending) ? 1u : 0u;
+    return P;
+}
+
+static lnt_batch::Group *group_of(lnt_batch *b, const lnt_ctx *c) {
+    for (auto &g : b->groups)
+        if (same_group(g.pl, c->pl) && g.ll == !c->prp) return &g;
+    return nullptr;
+}
+
+// Groups for the current members, each with room to grow (at least 8 slots, a power of
+// two, at least twice its members), their parameters, streams and the captured graph of
+// one iteration (every group a parallel branch forked from and joined to the batch stream).
 static int batch_build(lnt_batch *b) {
     batch_release(b);
     for (lnt_ctx *c : b->members) {
-        lnt_batch::Group *grp = nullptr;
-        for (auto &g : b->groups)
-            if (same_group(g.pl, c->pl) && g.ll == !c->prp) grp = &g;
+        lnt_batch::Group *grp = group_of(b, c);
         if (!grp) {
             b->groups.emplace_back();
             grp = &b->groups.back();
@@ -1036,12 +1057,13 @@ static int batch_build(lnt_batch *b) {
         grp->ctxs.push_back(c);
     }
     for (auto &g : b->groups) {
-        g.h.clear();
-        for (lnt_ctx *c : g.ctxs) {
-            LntParams P = c->P;
-            P.sub2 = (!c->prp && c->pending) ? 1u : 0u;
-            g.h.push_back(P);
-        }
+        u32 cap = 8;
+        while (cap < 2 * g.ctxs.size()) cap *= 2;
+        g.cap = cap;
+        g.ctxs.resize(cap, nullptr);
+        g.h.assign(cap, LntParams{});
+        for (u32 i = 0; i < cap; i++)
+            if (g.ctxs[i]) g.h[i] = member_params(g.ctxs[i]);
         BHIPCHK(b, hipMalloc(&g.d, g.h.size() * sizeof(LntParams)), "hipMalloc (batch parameters)");
         BHIPCHK(b, hipMemcpyAsync(g.d, g.h.data(), g.h.size() * sizeof(LntParams), hipMemcpyHostToDevice, b->stream),
                 "hipMemcpy (batch parameters)");
@@ -1049,7 +1071,6 @@ static int batch_build(lnt_batch *b) {
         BHIPCHK(b, hipEventCreateWithFlags(&g.join, hipEventDisableTiming), "hipEventCreate");
     }
     BHIPCHK(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
-    // capture: fork every group onto its own stream, join back
     BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     bool ok = true;
     if (hipEventRecord(b->fork, b->stream) != hipSuccess) ok = false;
@@ -1062,34 +1083,47 @@ static int batch_build(lnt_batch *b) {
     if (!ok || ec != hipSuccess) return bfail(b, LNT_DEVICE_ERROR, "batch graph capture failed");
     BHIPCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     b->dirty = false;
+    b->upload = false;
+    b->captures++;
     return LNT_OK;
 }
 
-// Each member's "-2" flag from its own state (set_digits, a product or a copy on a member
-// since the last batch step clears it): re-upload the parameters of groups that changed.
-static int batch_refresh_sub2(lnt_batch *b) {
-    for (auto &g : b->groups) {
-        bool changed = false;
-        for (size_t i = 0; i < g.ctxs.size(); i++) {
-            const u32 want = (!g.ctxs[i]->prp && g.ctxs[i]->pending) ? 1u : 0u;
-            if (g.h[i].sub2 != want) {
-                g.h[i].sub2 = want;
-                changed = true;
+// Copy the groups' parameter slots to the device (after members joined, left or changed).
+static int batch_upload(lnt_batch *b) {
+    for (auto &g : b->groups)
+        BHIPCHK(b, hipMemcpyAsync(g.d, g.h.data(), g.h.size() * sizeof(LntParams), hipMemcpyHostToDevice, b->stream),
+                "hipMemcpy (batch parameters)");
+    b->upload = false;
+    return LNT_OK;
+}
+
+// Each member's parameters from its own state: its "-2" flag (set_digits, a product or a
+// copy on a member since the last batch step clears it) and its buffers and tables (the
+// tuner may have re-planned it: a new kernel shape needs a new graph).
+static void batch_refresh(lnt_batch *b) {
+    for (auto &g : b->groups)
+        for (u32 i = 0; i < g.cap; i++) {
+            lnt_ctx *c = g.ctxs[i];
+            if (!c) continue;
+            if (!same_group(g.pl, c->pl)) {
+                b->dirty = true;
+                continue;
+            }
+            const LntParams P = member_params(c);
+            if (std::memcmp(&P, &g.h[i], sizeof P) != 0) {
+                g.h[i] = P;
+                b->upload = true;
             }
         }
-        if (changed)
-            BHIPCHK(b, hipMemcpyAsync(g.d, g.h.data(), g.h.size() * sizeof(LntParams), hipMemcpyHostToDevice, b->stream),
-                    "hipMemcpy (batch parameters)");
-    }
-    return LNT_OK;
 }
 
 // After an iteration every LL member owes its "-2": refresh the flags where they changed.
 static int batch_mark_pending(lnt_batch *b) {
     for (auto &g : b->groups) {
         bool changed = false;
-        for (size_t i = 0; i < g.ctxs.size(); i++) {
+        for (u32 i = 0; i < g.cap; i++) {
             lnt_ctx *c = g.ctxs[i];
+            if (!c) continue;
             c->pending = true;
             c->pristine = false;
             u32 want = c->prp ? 0u : 1u;
@@ -1130,7 +1164,16 @@ int lnt_batch_add(lnt_batch *b, lnt_ctx *c) {
     BHIPCHK(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
     b->members.push_back(c);
     c->batch = b;
-    b->dirty = true;
+    lnt_batch::Group *g = b->dirty ? nullptr : group_of(b, c);
+    u32 slot = 0;
+    while (g && slot < g->cap && g->ctxs[slot]) slot++;
+    if (g && slot < g->cap) {  // a free slot of an existing group: new parameters only
+        g->ctxs[slot] = c;
+        g->h[slot] = member_params(c);
+        b->upload = true;
+    } else {
+        b->dirty = true;
+    }
     return LNT_OK;
 }
 
@@ -1142,7 +1185,13 @@ int lnt_batch_remove(lnt_batch *b, lnt_ctx *c) {
         if (b->members[i] == c) {
             b->members.erase(b->members.begin() + i);
             c->batch = nullptr;
-            b->dirty = true;
+            for (auto &g : b->groups)  // its slot becomes empty
+                for (u32 j = 0; j < g.cap; j++)
+                    if (g.ctxs[j] == c) {
+                        g.ctxs[j] = nullptr;
+                        g.h[j] = LntParams{};
+                        b->upload = true;
+                    }
             return LNT_OK;
         }
     return bfail(b, LNT_INVALID_ARGUMENT, "context is not in the batch");
@@ -1150,17 +1199,23 @@ int lnt_batch_remove(lnt_batch *b, lnt_ctx *c) {
 
 int lnt_batch_size(const lnt_batch *b) { return b ? (int)b->members.size() : 0; }
 
+int lnt_batch_stats(const lnt_batch *b, uint64_t *out, int cap) {
+    if (!b || !out || cap < 3) return LNT_INVALID_ARGUMENT;
+    out[0] = b->members.size();
+    out[1] = b->groups.size();
+    out[2] = b->captures;
+    return LNT_OK;
+}
+
 int lnt_batch_iterate(lnt_batch *b, uint32_t k) {
     if (!b) return fail(nullptr, LNT_INVALID_ARGUMENT, "NULL batch");
     if (k == 0 || b->members.empty()) return LNT_OK;
     BHIPCHK(b, hipSetDevice(b->device), "hipSetDevice");
     for (lnt_ctx *c : b->members) BHIPCHK(b, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     int rc;
-    for (auto &g : b->groups)  // a member re-planned (lnt_tune) since the graph was built
-        for (size_t i = 0; i < g.ctxs.size() && !b->dirty; i++)
-            if (g.h[i].x.p != g.ctxs[i]->P.x.p || g.h[i].twr.p != g.ctxs[i]->P.twr.p) b->dirty = true;
+    if (!b->dirty) batch_refresh(b);
     if (b->dirty && (rc = batch_build(b))) return rc;
-    if ((rc = batch_refresh_sub2(b))) return rc;
+    if (b->upload && (rc = batch_upload(b))) return rc;
     // Members under a Gerbicz check count the squarings as checked iterations: the
     // replays stop at each of their block ends for the block step on the member's stream.
     // A batch of single-workgroup tests only runs `step` iterations in one launch per
@@ -1175,7 +1230,7 @@ int lnt_batch_iterate(lnt_batch *b, uint32_t k) {
             if (c->g.on) step = (uint32_t)std::min<uint64_t>(step, c->g.B - c->g.it % c->g.B);
         if (all_small) {
             for (auto &g : b->groups)
-                if (!lnt_launch_batch_small(g.pl, g.d, (u32)g.ctxs.size(), step, g.ll, b->stream))
+                if (!lnt_launch_batch_small(g.pl, g.d, g.cap, step, g.ll, b->stream))
                     return bfail(b, LNT_UNSUPPORTED, "no batched kernel for this shape");
             BHIPCHK(b, hipGetLastError(), "kernel launch");
             if (first && (rc = batch_mark_pending(b))) return rc;

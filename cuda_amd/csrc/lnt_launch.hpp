@@ -94,6 +94,7 @@ __global__ __launch_bounds__(B2 < 64 ? 64 : B2) void lnt_kb_small(const LntParam
     constexpr u32 W = (B1 * B2 > B1 * (B2 + LNT_PAD)) ? B1 * B2 : B1 * (B2 + LNT_PAD);
     __shared__ u64 smem[W];
     const LntParams P = Ps[blockIdx.x];
+    if (P.n == 0) return;  // an empty slot of the batch
     lnt_small_body<B1, B2, M>(threadIdx.x, LView{smem, W}, P, iters, P.sub2 != 0, ll != 0);
 }
 
@@ -102,6 +103,7 @@ __global__ __launch_bounds__(B * B) void lnt_kb_small3(const LntParams *__restri
     constexpr u32 W = lnt_small3_lds<B>();
     __shared__ u64 smem[W];
     const LntParams P = Ps[blockIdx.x];
+    if (P.n == 0) return;  // an empty slot of the batch
     lnt_small3_body<B, M>(threadIdx.x, LView{smem, W}, P, iters, P.sub2 != 0, ll != 0);
 }
 
@@ -111,6 +113,7 @@ __global__ __launch_bounds__(K * B2) void lnt_kb_col_fwd(const LntParams *__rest
     __shared__ u64 smem[W];
     const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
     const LntParams P = Ps[b / wgs];
+    if (P.n == 0) return;  // an empty slot of the batch
     lnt_col_fwd_body<B1, B2, K, M>(threadIdx.x, b % wgs, LView{smem, W}, P, P.sub2 != 0);
 }
 
@@ -120,6 +123,7 @@ __global__ __launch_bounds__(K * B2) void lnt_kb_col_inv(const LntParams *__rest
     __shared__ u64 smem[W];
     const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
     const LntParams P = Ps[b / wgs];
+    if (P.n == 0) return;  // an empty slot of the batch
     lnt_col_inv_body<B1, B2, K>(threadIdx.x, b % wgs, LView{smem, W}, P);
 }
 
@@ -129,6 +133,7 @@ __global__ __launch_bounds__(K * BB * BC) void lnt_kb_col3_fwd(const LntParams *
     __shared__ u64 smem[W];
     const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
     const LntParams P = Ps[b / wgs];
+    if (P.n == 0) return;  // an empty slot of the batch
     lnt_col3_fwd_body<BA, BB, BC, K, M>(threadIdx.x, b % wgs, LView{smem, W}, P, P.sub2 != 0);
 }
 
@@ -138,6 +143,7 @@ __global__ __launch_bounds__(K * BB * BC) void lnt_kb_col3_inv(const LntParams *
     __shared__ u64 smem[W];
     const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
     const LntParams P = Ps[b / wgs];
+    if (P.n == 0) return;  // an empty slot of the batch
     lnt_col3_inv_body<BA, BB, BC, K>(threadIdx.x, b % wgs, LView{smem, W}, P);
 }
 
@@ -146,6 +152,7 @@ __global__ __launch_bounds__(G * B2) void lnt_kb_row(const LntParams *__restrict
     constexpr u32 W = G * B1 * (B2 + LNT_PAD);
     __shared__ u64 smem[W];
     const LntParams P = Ps[blockIdx.x / wgs];
+    if (P.n == 0) return;  // an empty slot of the batch
     lnt_row_body<B1, B2, G>(threadIdx.x, blockIdx.x % wgs, LView{smem, W}, P);
 }
 
@@ -154,6 +161,7 @@ __global__ __launch_bounds__(G * B * B) void lnt_kb_row3(const LntParams *__rest
     constexpr u32 W = G * B * B * (B + 1);
     __shared__ u64 smem[W];
     const LntParams P = Ps[blockIdx.x / wgs];
+    if (P.n == 0) return;  // an empty slot of the batch
     lnt_row3_body<B, G>(threadIdx.x, blockIdx.x % wgs, LView{smem, W}, P);
 }
 
@@ -163,6 +171,7 @@ __global__ __launch_bounds__(BB * BC) __attribute__((amdgpu_waves_per_eu(BA == 1
     constexpr u32 W = LntRow3m<BA, BB, BC>::LDS;
     __shared__ u64 smem[W];
     const LntParams P = Ps[blockIdx.x / wgs];
+    if (P.n == 0) return;  // an empty slot of the batch
     lnt_row3m_body<BA, BB, BC>(threadIdx.x, blockIdx.x % wgs, LView{smem, W}, P);
 }
 

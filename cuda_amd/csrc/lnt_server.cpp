@@ -290,6 +290,12 @@ int lnt_server_stats(lnt_server *s, uint64_t *out, int cap) {
     out[4] = s->steps;
     out[5] = s->iterations_run;
     out[6] = s->admitted;
+    if (cap >= 8) {
+        uint64_t caps = 0, st[3];
+        for (int c = 0; c < CLASSES; c++)
+            if (lnt_batch_stats(s->batch[c], st, 3) == LNT_OK) caps += st[2];
+        out[7] = caps;
+    }
     return LNT_OK;
 }
 

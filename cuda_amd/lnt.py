@@ -36,7 +36,7 @@ EXPORTS = (
     "lnt_batch_destroy", "lnt_batch_last_error", "lnt_server_create", "lnt_server_submit",
     "lnt_server_poll", "lnt_server_stats", "lnt_server_now", "lnt_server_destroy",
     "lnt_server_last_error", "lnt_selftest", "lnt_tune_export", "lnt_tune_import",
-    "lnt_tune_results", "lnt_iterate_static", "lnt_valu_rates",
+    "lnt_tune_results", "lnt_iterate_static", "lnt_valu_rates", "lnt_batch_stats",
 )
 
 
@@ -107,6 +107,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lnt_batch_add.argtypes = [vp, vp]
     lib.lnt_batch_remove.argtypes = [vp, vp]
     lib.lnt_batch_size.argtypes = [vp]
+    lib.lnt_batch_stats.argtypes = [vp, u64p, ctypes.c_int]
     lib.lnt_batch_iterate.argtypes = [vp, u32]
     lib.lnt_batch_sync.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     lib.lnt_batch_destroy.argtypes = [vp]
@@ -369,6 +370,11 @@ class Batch:
     def __len__(self) -> int:
         return int(self._lib.lnt_batch_size(self._h))
 
+    def stats(self) -> dict:
+        out = (ctypes.c_uint64 * 3)()
+        self._check(self._lib.lnt_batch_stats(self._h, out, 3))
+        return {"members": int(out[0]), "groups": int(out[1]), "graph_captures": int(out[2])}
+
     def iterate(self, k: int) -> None:
         """Every member advances k iterations (asynchronous; a member's own later work is
         ordered after it)."""
@@ -421,9 +427,10 @@ class Server:
         return [{f: getattr(buf[i], f) for f, _ in Result._fields_} for i in range(k)]
 
     def stats(self) -> dict:
-        out = (ctypes.c_uint64 * 7)()
-        self._lib.lnt_server_stats(self._h, out, 7)
-        keys = ("queued", "running", "finished", "bytes_admitted", "steps", "iterations", "admitted")
+        out = (ctypes.c_uint64 * 8)()
+        self._lib.lnt_server_stats(self._h, out, 8)
+        keys = ("queued", "running", "finished", "bytes_admitted", "steps", "iterations", "admitted",
+                "graph_captures")
         return {k: int(v) for k, v in zip(keys, out)}
 
     def now(self) -> float:

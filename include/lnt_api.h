@@ -174,6 +174,9 @@ int lnt_batch_create(int device, lnt_batch **out);
 int lnt_batch_add(lnt_batch *batch, lnt_ctx *ctx);
 int lnt_batch_remove(lnt_batch *batch, lnt_ctx *ctx);
 int lnt_batch_size(const lnt_batch *batch);
+/* out[0] members, [1] kernel-shape groups, [2] iteration graphs captured so far (members
+ * joining or leaving a group with free slots only rewrite its parameters).  cap >= 3. */
+int lnt_batch_stats(const lnt_batch *batch, uint64_t *out, int cap);
 /* Every member advances k iterations (asynchronous). */
 int lnt_batch_iterate(lnt_batch *batch, uint32_t k);
 /* Wait; *elapsed_ms = GPU time since the previous lnt_batch_sync (or create). */
@@ -214,7 +217,8 @@ int lnt_server_submit(lnt_server *server, uint32_t p, int prp, uint64_t iteratio
  * the count (negative lnt_status on a bad argument). */
 int lnt_server_poll(lnt_server *server, lnt_result *out, int cap, int wait_ms);
 /* out[0] queued, [1] admitted and running, [2] finished, [3] device bytes admitted,
- * [4] scheduling steps, [5] iterations per member summed over steps, [6] admitted. cap >= 7. */
+ * [4] scheduling steps, [5] iterations per member summed over steps, [6] admitted,
+ * [7] (cap >= 8) iteration graphs captured by the server's batches.  cap >= 7. */
 int lnt_server_stats(lnt_server *server, uint64_t *out, int cap);
 double lnt_server_now(lnt_server *server);
 /* Stop the scheduler; tests still running end with status LNT_INVALID_ARGUMENT. */

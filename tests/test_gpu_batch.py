@@ -202,3 +202,34 @@ def test_member_state_changed_between_batch_steps():
     for t, s, what in ((a, sa, "set_digits"), (b, sb, "mul"), (c, sc, "copy")):
         assert t.get_digits() == s.get_digits(), what
     batch.close()
+
+
+def test_membership_changes_reuse_the_graph():
+    """Members joining or leaving a kernel-shape group with free slots rewrite its parameter
+    slots; the iteration graph is captured again only for a new shape (or a full group).
+    Every member still equals its solo run."""
+    p1, p2 = 1000003, 1000033  # both 2^16
+    batch = cuda_amd.Batch()
+    a, b, c = cuda_amd.LucasPRP(p1), cuda_amd.LucasPRP(p2), cuda_amd.LucasPRP(p1, prp=True)
+    sa, sb, sc = cuda_amd.LucasPRP(p1), cuda_amd.LucasPRP(p2), cuda_amd.LucasPRP(p1, prp=True)
+    for t in (a, b):
+        batch.add(t)
+    batch.iterate(5)
+    assert batch.stats()["graph_captures"] == 1
+    batch.remove(a)
+    d = cuda_amd.LucasPRP(p2)
+    sd = cuda_amd.LucasPRP(p2)
+    batch.add(d)  # same shape: a free slot
+    batch.iterate(6)
+    assert batch.stats()["graph_captures"] == 1
+    batch.add(c)  # PRP: another group, a new graph
+    batch.iterate(7)
+    assert batch.stats()["graph_captures"] == 2
+    sa.iterate(5)
+    sb.iterate(18)
+    sd.iterate(13)
+    sc.iterate(7)
+    a.sync()
+    for t, s, what in ((a, sa, "left"), (b, sb, "stayed"), (d, sd, "joined"), (c, sc, "new shape")):
+        assert t.get_digits() == s.get_digits(), what
+    batch.close()
