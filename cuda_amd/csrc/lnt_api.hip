@@ -99,8 +99,8 @@ static int build_tables(lnt_ctx *c) {
     const size_t nb = (size_t)pl.n * 8;
     HIPCHK(c, hipMalloc(&c->x, nb), "hipMalloc (x)");
     HIPCHK(c, hipMalloc(&c->y, nb), "hipMalloc (work buffer)");
-    HIPCHK(c, hipMalloc(&c->w, nb), "hipMalloc (weights)");
-    HIPCHK(c, hipMalloc(&c->uw, nb), "hipMalloc (unweights)");
+    HIPCHK(c, hipMalloc(&c->w, (size_t)z.W * 8), "hipMalloc (weights)");
+    HIPCHK(c, hipMalloc(&c->uw, (size_t)z.W * 8), "hipMalloc (unweights)");
     HIPCHK(c, hipMalloc(&c->twr, (size_t)z.Lr * 8), "hipMalloc (row twiddles)");
     HIPCHK(c, hipMalloc(&c->itwr, (size_t)z.Lr * 8), "hipMalloc (row twiddles)");
     if (z.large) {
@@ -618,37 +618,57 @@ int lnt_profile(lnt_ctx *c, uint32_t iters, char *names, size_t names_len, float
     const Plan &pl = c->pl;
     const bool ll = !c->prp;
     const int nk = pl.small ? 1 : 3;
-    // Each kernel runs `iters` times back to back between two events on the context's
-    // stream (the residue is saved first and restored after: repeated launches of one
-    // kernel are timing only).  Events between the kernels of every iteration would add
-    // their own gaps to each duration; back-to-back launches agree with the kernel trace.
+    // Each kernel runs `iters` times back to back as one captured graph, the way the
+    // iteration graph runs it (no host launch gap between dispatches and no event packet
+    // between kernels), timed by two events around the graph launch.  The residue is saved
+    // first and restored after: repeated launches of one kernel are timing only.
     u64 *save = nullptr;
     HIPCHK(c, hipMalloc(&save, (size_t)pl.n * 8), "hipMalloc (profile)");
     HIPCHK(c, hipMemcpyAsync(save, c->x, (size_t)pl.n * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpy");
-    std::vector<hipEvent_t> ev((size_t)nk * 2);
-    for (auto &e : ev) HIPCHK(c, hipEventCreate(&e), "hipEventCreate");
+    const LntParams P = params_for(c, c->x, c->y, nullptr);
     const bool sub2 = ll && c->pending;
-    for (int k = 0; k < nk; k++) {
-        HIPCHK(c, hipEventRecord(ev[2 * k], c->stream), "hipEventRecord");
-        for (uint32_t i = 0; i < iters; i++) {
-            if (pl.small) lnt_launch_small(pl, c->P, 1, sub2, ll, c->stream);
-            else if (k == 0) lnt_launch_col_fwd(pl, c->P, sub2, c->stream);
-            else if (k == 1) lnt_launch_row(pl, c->P, c->stream);
-            else lnt_launch_col_inv(pl, c->P, c->stream);
-        }
-        HIPCHK(c, hipEventRecord(ev[2 * k + 1], c->stream), "hipEventRecord");
-    }
-    HIPCHK(c, hipGetLastError(), "kernel launch");
-    HIPCHK(c, hipMemcpyAsync(c->x, save, (size_t)pl.n * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpy");
-    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-    (void)hipFree(save);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     std::vector<double> tot(nk, 0.0);
-    for (int k = 0; k < nk; k++) {
-        float t = 0;
-        HIPCHK(c, hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]), "hipEventElapsedTime");
-        tot[k] = t;
+    int rc = LNT_OK;
+    std::string why;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = LNT_DEVICE_ERROR;
+        why = "hipEventCreate";
     }
-    for (auto &e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < nk && rc == LNT_OK; k++) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        bool ok = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        for (uint32_t i = 0; i < iters && ok; i++) {
+            if (pl.small) ok = lnt_launch_small(pl, P, 1, sub2, ll, c->stream);
+            else if (k == 0) ok = lnt_launch_col_fwd(pl, P, sub2, c->stream);
+            else if (k == 1) ok = lnt_launch_row(pl, P, c->stream);
+            else ok = lnt_launch_col_inv(pl, P, c->stream);
+        }
+        const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+        float t = 0;
+        if (!ok || ec != hipSuccess || hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+            rc = ok ? LNT_DEVICE_ERROR : LNT_UNSUPPORTED;
+            why = "profile graph capture";
+        } else if (hipGraphLaunch(ge, c->stream) != hipSuccess ||  // warm-up
+                   hipEventRecord(e0, c->stream) != hipSuccess || hipGraphLaunch(ge, c->stream) != hipSuccess ||
+                   hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                   hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+            rc = LNT_DEVICE_ERROR;
+            why = "profile graph launch";
+        }
+        tot[k] = t;
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    const hipError_t ce = hipMemcpyAsync(c->x, save, (size_t)pl.n * 8, hipMemcpyDeviceToDevice, c->stream);
+    const hipError_t se = hipStreamSynchronize(c->stream);
+    (void)hipFree(save);
+    if (rc != LNT_OK) return fail(c, rc, why);
+    HIPCHK(c, ce, "hipMemcpy");
+    HIPCHK(c, se, "hipStreamSynchronize");
     std::string d = describe(pl);
     std::vector<std::string> parts;
     size_t pos = 0;
@@ -1323,8 +1343,8 @@ bool static_layout(const Plan &pl, u64 *base, TablePtrs &t) {
         return ptr;
     };
     t.y = take(z.n);
-    t.w = take(z.n);
-    t.uw = take(z.n);
+    t.w = take(z.W);
+    t.uw = take(z.W);
     t.twr = take(z.Lr);
     t.itwr = take(z.Lr);
     if (z.large) {

@@ -44,8 +44,10 @@
 struct LntParams {
     GView<u64> x;            // n words: residue state (pre-carry values between iterations)
     GView<u64> y;            // n words: transform-domain work buffer
-    GView<const u64> w, uw;  // IBDWT weights, unweights * n^-1 (natural digit order)
-    GView<const u64> tw4, itw4;  // 4-step twiddles (large path), natural slots
+    GView<const u64> w, uw;  // IBDWT weights, unweights * n^-1: n words in digit order (one-workgroup
+                             // path); 2R row factors indexed by lnt_wslot (large path)
+    GView<const u64> tw4, itw4;  // 4-step twiddles (large path), natural slots, times the column
+                                 // weight factor (unweight factor * n^-1)
     GView<const u64> twr, itwr;  // row-line twiddles (C words)
     GView<const u64> twr2, itwr2;  // second-level row twiddles (3-phase rows: B^2 words)
     GView<const u64> twc, itwc;  // column-line twiddles (R words)
@@ -59,6 +61,19 @@ struct LntParams {
 // e_g = ceil(p g / n) for 0 <= g <= n (bit offset of digit g).
 GL_HD u32 lnt_digit_offset(u32 p, u32 log_n, u64 g) {
     return (u32)(((u64)p * g + ((1ull << log_n) - 1)) >> log_n);
+}
+
+// Weight exponents on the large path.  Digit i = c + C r has weight B^e_i, e_i = (-p i) mod n
+// (B^n = 2), and e_i = e_c + f_r - n [e_c + f_r >= n] with e_c = (-p c) mod n and
+// f_r = (-p C r) mod n.  So w_i = B^e_c * B^f_r * 2^-[wrap]: the column factor B^e_c is
+// constant down a column and commutes with the column transform, so the 4-step twiddles
+// carry it; the row factor B^f_r, or B^f_r / 2 on a wrap, sits in a table of 2R words at
+// slot 2r + wrap (unweights likewise: B^-f_r and 2 B^-f_r, with B^-e_c n^-1 in itw4).
+GL_HD u32 lnt_wexp(u32 p, u32 n, u64 i) { return (0u - (u32)((u64)p * i)) & (n - 1); }
+// mC = -p C mod 2^32 (so f_r = (mC r) mod n, n being a power of two), ec = e_c.
+GL_HD u32 lnt_wslot(u32 ec, u32 mC, u32 n, u32 r) {
+    const u32 fr = (mC * r) & (n - 1);
+    return 2 * r + (ec + fr >= n ? 1u : 0u);
 }
 
 // Balanced split of v by width w: v = lo + hi * 2^w, -2^(w-1) <= lo < 2^(w-1).
@@ -139,6 +154,7 @@ LNT_KFN void lnt_col_fwd_body(int tid, int bid, LView lds, const LntParams &P, b
     const u64 c0 = (u64)bid * K;
     const int cl = tid % K, t = tid / K;
     const u64 c = c0 + cl;
+    const u32 ec = lnt_wexp(P.p, P.n, c), mC = 0u - P.p * P.C;  // weight slot parameters
 
     // 0. loads: the tile (K adjacent lanes read one row run: 8K contiguous bytes), the M
     //    digits before each of this thread's carry rows, and the weights of phase 1.
@@ -159,7 +175,7 @@ LNT_KFN void lnt_col_fwd_body(int tid, int bid, LView lds, const LntParams &P, b
     }
     if (PF) {
 #pragma unroll
-        for (int j = 0; j < B1; j++) wv[PF ? j : 0] = P.w[c + C * (u64)(t + B2 * j)];
+        for (int j = 0; j < B1; j++) wv[PF ? j : 0] = P.w[lnt_wslot(ec, mC, P.n, (u32)(t + B2 * j))];
     }
 #pragma unroll
     for (int j = 0; j < B1; j++) lds[(u32)((t + B2 * j) * KS + cl)] = a[j];
@@ -185,7 +201,8 @@ LNT_KFN void lnt_col_fwd_body(int tid, int bid, LView lds, const LntParams &P, b
     // 2. phase 1: weight, B1-point DFT down the column, inter-block twiddle
 #pragma unroll
     for (int j = 0; j < B1; j++)
-        a[j] = gl_mul(lds[(u32)((t + B2 * j) * KS + cl)], PF ? wv[PF ? j : 0] : P.w[c + C * (u64)(t + B2 * j)]);
+        a[j] = gl_mul(lds[(u32)((t + B2 * j) * KS + cl)],
+                      PF ? wv[PF ? j : 0] : P.w[lnt_wslot(ec, mC, P.n, (u32)(t + B2 * j))]);
     dft_fwd<B1>(a);
     if (B2 > 1) {
 #pragma unroll
@@ -218,6 +235,7 @@ LNT_KFN void lnt_col_inv_body(int tid, int bid, LView lds, const LntParams &P) {
     const u64 C = P.C;
     const int cl = tid % K, t = tid / K;
     const u64 c = (u64)bid * K + cl;
+    const u32 ec = lnt_wexp(P.p, P.n, c), mC = 0u - P.p * P.C;  // weight slot parameters
     constexpr bool PF = B1 <= 32;  // prefetch the unweighting factors (fits when B1 <= 32)
     u64 a[B1], uv[PF ? B1 : 1];
     // (the inverse 4-step twiddle was applied by the row pass as it stored the row)
@@ -225,7 +243,7 @@ LNT_KFN void lnt_col_inv_body(int tid, int bid, LView lds, const LntParams &P) {
     for (int i = 0; i < B1; i++) a[i] = P.y[c + C * ((u64)t * B1 + i)];
     if (PF) {
 #pragma unroll
-        for (int j = 0; j < B1; j++) uv[PF ? j : 0] = P.uw[c + C * ((u64)t + (u64)B2 * j)];
+        for (int j = 0; j < B1; j++) uv[PF ? j : 0] = P.uw[lnt_wslot(ec, mC, P.n, (u32)(t + B2 * j))];
     }
 #pragma unroll
     for (int q = 0; q < RP; q++) {
@@ -244,7 +262,7 @@ LNT_KFN void lnt_col_inv_body(int tid, int bid, LView lds, const LntParams &P) {
 #pragma unroll
     for (int j = 0; j < B1; j++) {
         const u64 idx = c + C * ((u64)t + (u64)B2 * j);
-        P.x[idx] = gl_mul(a[j], PF ? uv[PF ? j : 0] : P.uw[idx]);
+        P.x[idx] = gl_mul(a[j], PF ? uv[PF ? j : 0] : P.uw[lnt_wslot(ec, mC, P.n, (u32)(t + B2 * j))]);
     }
 }
 
@@ -280,6 +298,7 @@ LNT_KFN void lnt_col3_fwd_body(int tid, int bid, LView lds, const LntParams &P, 
     const u64 c0 = (u64)bid * K;
     const int cl = tid % K, t = tid / K;
     const u64 c = c0 + cl;
+    const u32 ec = lnt_wexp(P.p, P.n, c), mC = 0u - P.p * P.C;  // weight slot parameters
 
     // 0. loads: rows t + A j of column c (K adjacent lanes read one row run), the halo
     //    digits of this thread's carry rows, the weights (prefetched into registers only
@@ -299,7 +318,7 @@ LNT_KFN void lnt_col3_fwd_body(int tid, int bid, LView lds, const LntParams &P, 
     }
     if (PW) {
 #pragma unroll
-        for (int j = 0; j < BA; j++) wv[PW ? j : 0] = P.w[c + C * (u64)(t + A * j)];
+        for (int j = 0; j < BA; j++) wv[PW ? j : 0] = P.w[lnt_wslot(ec, mC, P.n, (u32)(t + A * j))];
     }
 #pragma unroll
     for (int j = 0; j < BA; j++) lds[(u32)((t + A * j) * KS + cl)] = a[j];
@@ -336,7 +355,8 @@ LNT_KFN void lnt_col3_fwd_body(int tid, int bid, LView lds, const LntParams &P, 
     // 2. phase 1: weight, BA-point DFT, twiddle
 #pragma unroll
     for (int j = 0; j < BA; j++)
-        a[j] = gl_mul(lds[(u32)((t + A * j) * KS + cl)], PW ? wv[PW ? j : 0] : P.w[c + C * (u64)(t + A * j)]);
+        a[j] = gl_mul(lds[(u32)((t + A * j) * KS + cl)],
+                      PW ? wv[PW ? j : 0] : P.w[lnt_wslot(ec, mC, P.n, (u32)(t + A * j))]);
     dft_fwd<BA>(a);
 #pragma unroll
     for (int rho = 1; rho < BA; rho++) a[rho] = gl_mul(a[rho], P.twc[(u64)rho * A + t]);
@@ -380,11 +400,12 @@ LNT_KFN void lnt_col3_inv_body(int tid, int bid, LView lds, const LntParams &P) 
     const u64 C = P.C;
     const int cl = tid % K, t = tid / K;
     const u64 c = (u64)bid * K + cl;
+    const u32 ec = lnt_wexp(P.p, P.n, c), mC = 0u - P.p * P.C;  // weight slot parameters
     u64 a[BA], uv[BA];
 #pragma unroll
     for (int i = 0; i < BA; i++) a[i] = P.y[c + C * ((u64)t * BA + i)];
 #pragma unroll
-    for (int j = 0; j < BA; j++) uv[j] = P.uw[c + C * ((u64)t + (u64)A * j)];
+    for (int j = 0; j < BA; j++) uv[j] = P.uw[lnt_wslot(ec, mC, P.n, (u32)(t + A * j))];
     // phase 3 inverse, then the inverse phase-2 twiddle
 #pragma unroll
     for (int q = 0; q < S::QC; q++) {
@@ -945,6 +966,17 @@ GL_HD u64 lnt_tw4_3(u64 omega_n, u32 C, u32 BA, u32 BB, u32 BC, u64 idx) {
     for (u32 b = 1, v = rho; b < BC; b <<= 1, v >>= 1) br = (br << 1) | (v & 1);
     const u64 kcol = k1 + (u64)BA * (k2 + (u64)BB * br);
     return gl_pow(omega_n, (u64)c * kcol);
+}
+
+// Large-path weight tables (the factorisation at lnt_wexp): row slot 2r + h holds B^f_r,
+// times m when h = 1 (B = weight base, m = 1/2; or B = its inverse, m = 2 for unweights).
+GL_HD u64 lnt_row_factor(u64 B, u64 m, u32 p, u32 n, u32 C, u32 slot) {
+    const u64 v = gl_pow(B, lnt_wexp(p, n, (u64)C * (slot >> 1)));
+    return (slot & 1) ? gl_mul(v, m) : v;
+}
+// Column factor B^e_c times s (s = 1 for weights, n^-1 for unweights), c = idx mod C.
+GL_HD u64 lnt_col_factor(u64 B, u64 s, u32 p, u32 n, u32 C, u64 idx) {
+    return gl_mul(gl_pow(B, lnt_wexp(p, n, idx % C)), s);
 }
 
 // IBDWT weight of digit i (common/pre_calculation.cu:23-36): B^(n - (p i mod n)) for

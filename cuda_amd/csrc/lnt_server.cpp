@@ -54,12 +54,13 @@ int size_class(uint32_t p) {
     return l <= 12 ? 0 : l <= 17 ? 1 : 2;
 }
 
-// Device bytes of one test: residue, work buffer, weights, unweights, the two 4-step
-// twiddle tables and the product operand (8 bytes per word each), six residue slots
-// under the Gerbicz check, and the line tables.
+// Device bytes of one test: residue, work buffer, the two 4-step twiddle tables and the
+// product operand (8 bytes per word each), six residue slots under the Gerbicz check, and
+// the line and weight tables (weights take 2R words each from 2^13 digits on, n words
+// each below, within the 256 KiB allowance).
 uint64_t footprint(uint32_t p, bool gz) {
     const uint64_t n = 1ull << std::max<uint32_t>(3, lnt_secured_log2_length(p));
-    return 8 * n * (7 + (gz ? 6 : 0)) + (256u << 10);
+    return 8 * n * (5 + (gz ? 6 : 0)) + (256u << 10);
 }
 
 }  // namespace

@@ -25,22 +25,30 @@ static __global__ void lnt_k_weights(u64 Bw, u64 Binv, u64 ninv, u32 p, u32 n, u
     uw[i] = gl_mul(gl_pow(Binv, (u64)(n - r)), ninv);
 }
 
+static __global__ void lnt_k_row_weights(u64 Bw, u64 Binv, u32 p, u32 n, u32 C, u32 L, u64 *w, u64 *uw) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L) return;
+    w[i] = lnt_row_factor(Bw, (GL_P + 1) / 2, p, n, C, i);
+    uw[i] = lnt_row_factor(Binv, 2, p, n, C, i);
+}
+
 static __global__ void lnt_k_line_tw(u64 om, u32 B1, u32 B2, u32 L, u64 *tw) {
     u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= L) return;
     tw[i] = lnt_line_tw(om, B1, B2, i);
 }
 
-static __global__ void lnt_k_tw4(u64 om, u32 C, u32 B1, u32 B2, u32 n, u64 *tw) {
+// 4-step twiddles times the column weight factor B^e_c * s (lnt_col_factor)
+static __global__ void lnt_k_tw4(u64 om, u32 C, u32 B1, u32 B2, u32 n, u64 B, u64 s, u32 p, u64 *tw) {
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    tw[i] = lnt_tw4(om, C, B1, B2, i);
+    tw[i] = gl_mul(lnt_tw4(om, C, B1, B2, i), lnt_col_factor(B, s, p, n, C, i));
 }
 
-static __global__ void lnt_k_tw4_3(u64 om, u32 C, u32 BA, u32 BB, u32 BC, u32 n, u64 *tw) {
+static __global__ void lnt_k_tw4_3(u64 om, u32 C, u32 BA, u32 BB, u32 BC, u32 n, u64 B, u64 s, u32 p, u64 *tw) {
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    tw[i] = lnt_tw4_3(om, C, BA, BB, BC, i);
+    tw[i] = gl_mul(lnt_tw4_3(om, C, BA, BB, BC, i), lnt_col_factor(B, s, p, n, C, i));
 }
 
 template <class T>
@@ -51,6 +59,7 @@ static GView<T> view(T *p, u64 n) {
 // Word counts of the tables of a plan (0 where the plan has no such table).
 struct TableSizes {
     u32 n, Lr, L2, R, A;  // residue, row line, second-level row line, column line, second-level column line
+    u32 W;                // each weight table: n digit weights (one workgroup) or 2R row factors
     bool large;
 };
 static TableSizes table_sizes(const Plan &pl) {
@@ -61,6 +70,7 @@ static TableSizes table_sizes(const Plan &pl) {
     t.L2 = pl.row3m ? pl.rBB * pl.rBC : pl.row3 ? pl.row3 * pl.row3 : 0;
     t.R = pl.small ? 0 : pl.R;
     t.A = (!pl.small && pl.col3) ? pl.R / pl.col3 : 0;
+    t.W = pl.small ? pl.n : 2 * pl.R;
     return t;
 }
 
@@ -76,8 +86,12 @@ static void fill_tables(const Plan &pl, u32 p, const TablePtrs &t, hipStream_t s
     const TableSizes z = table_sizes(pl);
     const u32 n = pl.n;
     const u64 Bw = gl_pow(7, (GL_P - 1) / 192 / n * 5);
-    const u64 ninv = gl_inv(n);
-    hipLaunchKernelGGL(lnt_k_weights, dim3((n + 255) / 256), dim3(256), 0, s, Bw, gl_inv(Bw), ninv, p, n, t.w, t.uw);
+    const u64 ninv = gl_inv(n), Binv = gl_inv(Bw);
+    if (z.large)
+        hipLaunchKernelGGL(lnt_k_row_weights, dim3((z.W + 255) / 256), dim3(256), 0, s, Bw, Binv, p, n, pl.C, z.W, t.w,
+                           t.uw);
+    else
+        hipLaunchKernelGGL(lnt_k_weights, dim3((n + 255) / 256), dim3(256), 0, s, Bw, Binv, ninv, p, n, t.w, t.uw);
     const u32 Lr = z.Lr;
     u64 omr = gl_pow(554, (GL_P - 1) / Lr);
     // row line twiddles: first phase B1 x B2 (3-phase rows: row3 x row3^2, row3m x rBB rBC)
@@ -99,18 +113,18 @@ static void fill_tables(const Plan &pl, u32 p, const TablePtrs &t, hipStream_t s
             hipLaunchKernelGGL(lnt_k_line_tw, dim3((A + 255) / 256), dim3(256), 0, s, gl_inv(oma), pl.cBB, pl.cBC, A,
                                t.itwc2);
             hipLaunchKernelGGL(lnt_k_tw4_3, dim3((n + 255) / 256), dim3(256), 0, s, omn, pl.C, pl.col3, pl.cBB, pl.cBC, n,
-                               t.tw4);
+                               Bw, (u64)1, p, t.tw4);
             hipLaunchKernelGGL(lnt_k_tw4_3, dim3((n + 255) / 256), dim3(256), 0, s, gl_inv(omn), pl.C, pl.col3, pl.cBB,
-                               pl.cBC, n, t.itw4);
+                               pl.cBC, n, Binv, ninv, p, t.itw4);
         } else {
             hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, s, omc, pl.col.B1, pl.col.B2, pl.R,
                                t.twc);
             hipLaunchKernelGGL(lnt_k_line_tw, dim3((pl.R + 255) / 256), dim3(256), 0, s, gl_inv(omc), pl.col.B1,
                                pl.col.B2, pl.R, t.itwc);
             hipLaunchKernelGGL(lnt_k_tw4, dim3((n + 255) / 256), dim3(256), 0, s, omn, pl.C, pl.col.B1, pl.col.B2, n,
-                               t.tw4);
+                               Bw, (u64)1, p, t.tw4);
             hipLaunchKernelGGL(lnt_k_tw4, dim3((n + 255) / 256), dim3(256), 0, s, gl_inv(omn), pl.C, pl.col.B1,
-                               pl.col.B2, n, t.itw4);
+                               pl.col.B2, n, Binv, ninv, p, t.itw4);
         }
     }
     // second-phase row twiddles: a line of L2 = B2 words split (row3 x row3 or rBB x rBC)
@@ -130,8 +144,8 @@ static LntParams params_of(const Plan &pl, u32 p, u64 *x, const TablePtrs &t) {
     LntParams P{};
     P.x = view<u64>(x, n);
     P.y = view<u64>(t.y, n);
-    P.w = view<const u64>(t.w, n);
-    P.uw = view<const u64>(t.uw, n);
+    P.w = view<const u64>(t.w, z.W);
+    P.uw = view<const u64>(t.uw, z.W);
     P.tw4 = view<const u64>(t.tw4, z.large ? n : 0);
     P.itw4 = view<const u64>(t.itw4, z.large ? n : 0);
     P.twr = view<const u64>(t.twr, z.Lr);

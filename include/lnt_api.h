@@ -116,9 +116,10 @@ uint32_t lnt_length(const lnt_ctx *ctx);
 int lnt_describe(const lnt_ctx *ctx, char *buf, size_t buflen);
 
 /* Per-kernel average GPU durations (ms): each kernel of the iteration is launched `iters`
- * times back to back between two events on the context's stream; the residue is saved
- * before and restored after, so the test's state is unchanged.  names/ms receive up to
- * `cap` entries; returns the count. */
+ * times back to back as one captured graph (as the iteration graph runs it), timed by two
+ * events around the graph launch on the context's stream; the residue is saved before and
+ * restored after, so the test's state is unchanged.  names/ms receive up to `cap`
+ * entries; returns the count. */
 int lnt_profile(lnt_ctx *ctx, uint32_t iters, char *names, size_t names_len, float *ms, int cap);
 
 /* ---- products, copies and the Gerbicz-checked PRP (new: the reference has no error

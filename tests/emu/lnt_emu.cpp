@@ -60,13 +60,23 @@ extern "C" void *emu_create3(uint32_t p, uint32_t n, int prp, uint32_t lc, uint3
     e->x.assign(n, 0);
     e->y.assign(n, 0);
     e->f.assign(n, 0);
-    e->w.resize(n);
-    e->uw.resize(n);
     u64 Bw = gl_pow(7, (GL_P - 1) / 192 / n * 5);
+    u64 Binv = gl_inv(Bw);
     u64 ninv = gl_inv(n);
-    for (uint32_t i = 0; i < n; i++) {
-        e->w[i] = lnt_weight(Bw, p, n, i);
-        e->uw[i] = gl_mul(gl_inv(e->w[i]), ninv);
+    if (pl.small) {
+        e->w.resize(n);
+        e->uw.resize(n);
+        for (uint32_t i = 0; i < n; i++) {
+            e->w[i] = lnt_weight(Bw, p, n, i);
+            e->uw[i] = gl_mul(gl_inv(e->w[i]), ninv);
+        }
+    } else {  // row factors; the column factors go into tw4 / itw4 below
+        e->w.resize(2 * pl.R);
+        e->uw.resize(2 * pl.R);
+        for (uint32_t i = 0; i < 2 * pl.R; i++) {
+            e->w[i] = lnt_row_factor(Bw, (GL_P + 1) / 2, p, n, pl.C, i);
+            e->uw[i] = lnt_row_factor(Binv, 2, p, n, pl.C, i);
+        }
     }
     widths(p, n, e->wid);
     if (pl.row3m) {
@@ -103,8 +113,9 @@ extern "C" void *emu_create3(uint32_t p, uint32_t n, int prp, uint32_t lc, uint3
         e->itw4.resize(n);
         u64 om = gl_pow(554, (GL_P - 1) / n), iom = gl_inv(om);
         for (uint64_t i = 0; i < n; i++) {
-            e->tw4[i] = lnt_tw4_3(om, pl.C, pl.col3, pl.cBB, pl.cBC, i);
-            e->itw4[i] = lnt_tw4_3(iom, pl.C, pl.col3, pl.cBB, pl.cBC, i);
+            e->tw4[i] = gl_mul(lnt_tw4_3(om, pl.C, pl.col3, pl.cBB, pl.cBC, i), lnt_col_factor(Bw, 1, p, n, pl.C, i));
+            e->itw4[i] =
+                gl_mul(lnt_tw4_3(iom, pl.C, pl.col3, pl.cBB, pl.cBC, i), lnt_col_factor(Binv, ninv, p, n, pl.C, i));
         }
     } else if (!pl.small) {
         line_tables(pl.R, pl.col, e->twc, e->itwc);
@@ -112,8 +123,8 @@ extern "C" void *emu_create3(uint32_t p, uint32_t n, int prp, uint32_t lc, uint3
         e->itw4.resize(n);
         u64 om = gl_pow(554, (GL_P - 1) / n), iom = gl_inv(om);
         for (uint64_t i = 0; i < n; i++) {
-            e->tw4[i] = lnt_tw4(om, pl.C, pl.col.B1, pl.col.B2, i);
-            e->itw4[i] = lnt_tw4(iom, pl.C, pl.col.B1, pl.col.B2, i);
+            e->tw4[i] = gl_mul(lnt_tw4(om, pl.C, pl.col.B1, pl.col.B2, i), lnt_col_factor(Bw, 1, p, n, pl.C, i));
+            e->itw4[i] = gl_mul(lnt_tw4(iom, pl.C, pl.col.B1, pl.col.B2, i), lnt_col_factor(Binv, ninv, p, n, pl.C, i));
         }
     } else {
         e->twc.assign(1, 1);

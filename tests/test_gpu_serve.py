@@ -66,8 +66,8 @@ def test_served_equals_alone_with_staggered_arrivals():
 
 
 def test_budget_limits_admission():
-    # room for two 2^16-word tests at a time (7 tables of 512 KiB each + slack)
-    srv = Server(hbm_budget=2 * (8 * 65536 * 7 + (256 << 10)), step_ms=1.0, gerbicz_block=0)
+    # room for two 2^16-word tests at a time (5 words per digit of 8 bytes + 256 KiB each)
+    srv = Server(hbm_budget=2 * (8 * 65536 * 5 + (256 << 10)), step_ms=1.0, gerbicz_block=0)
     ps = [1000003, 1000033, 1000037, 1000039, 1000081]
     for i, p in enumerate(ps):
         srv.submit(p, False, 400, i)
