@@ -216,17 +216,20 @@ def secured(p: int) -> int:
     return secured_log2_length(p)
 
 
-def counters_for(n: int, name: str):
-    """HBM bytes and VALU lane-instructions per launch of kernel `name` at length n, from the
-    rocprofv3 counter runs summarised in profiles/ (exact kernel name and length only)."""
+def counter_doc(n: int):
+    """The rocprofv3 counter summary in profiles/ when it was taken at length n, else {}."""
     path = os.path.join(ROOT, "profiles", "traffic_2p22.json")
     try:
         doc = json.load(open(path))
     except (OSError, ValueError):
         return {}
-    if doc.get("n") != n:
-        return {}
-    return doc.get("kernels", {}).get(name, {})
+    return doc if doc.get("n") == n else {}
+
+
+def counters_for(n: int, name: str):
+    """HBM bytes and VALU lane-instructions per launch of kernel `name` at length n, from the
+    rocprofv3 counter runs summarised in profiles/ (exact kernel name and length only)."""
+    return counter_doc(n).get("kernels", {}).get(name, {})
 
 
 def main() -> int:
@@ -293,8 +296,8 @@ def main() -> int:
     g = golden_res64(p, mode)
     check = (g[done] == res64) if done in g else None
 
-    # per-kernel durations: each kernel `profile_iters` times back to back between two HIP
-    # events on the test's stream (the residue is saved and restored around it)
+    # per-kernel durations: each kernel `profile_iters` times back to back as one captured
+    # graph, timed by two HIP events on the test's stream (the residue is saved and restored)
     prof = t.profile(args.profile_iters)
     t.close()
     name, kms = max(prof, key=lambda kv: kv[1])
@@ -305,12 +308,23 @@ def main() -> int:
     ghz = rates.pop("ghz")
     counters = counters_for(n, name)
     valu = None
-    if counters.get("valu_lane_ops"):
-        peak = 256 * LANES_PER_CU_CLOCK * ghz * 1e9
+    # VALU roof: the best issue rate the field-operation probes reach (operations per CU per
+    # measured clock x their VALU instructions per operation, counted by SQ_INSTS_VALU), in
+    # wave64 instructions per CU per clock; 32-bit operations issue faster than one per
+    # quad-cycle per SIMD on gfx950, so this exceeds 1 for the mixed 32/64-bit field code
+    probe_instr = counter_doc(n).get("valu_probe_instr_per_op", {})
+    issue = {k: rates[k] * probe_instr[k] / LANES_PER_CU_CLOCK
+             for k in rates if k in probe_instr and k != "v_mad_u64_u32"}
+    if counters.get("valu_lane_ops") and issue:
+        roof_op = max(issue, key=issue.get)
+        peak = 256 * LANES_PER_CU_CLOCK * issue[roof_op] * ghz * 1e9
         va = counters["valu_lane_ops"] / (kms * 1e-3)
         valu = {"achieved": round(va / 1e12, 2), "peak": round(peak / 1e12, 2), "unit": "T lane-instr/s",
                 "frac": round(va / peak, 4), "lane_instr_per_launch": counters["valu_lane_ops"],
-                "clock_ghz_measured": round(ghz, 3)}
+                "clock_ghz_measured": round(ghz, 3),
+                "peak_basis": f"{roof_op} probe: {issue[roof_op]:.3f} wave64 VALU instructions per CU per clock",
+                "wave_instr_per_cu_clock": round(va / (256 * LANES_PER_CU_CLOCK * ghz * 1e9), 3),
+                "probe_wave_instr_per_cu_clock": {k: round(v, 3) for k, v in issue.items()}}
     line = {
         "metric": METRIC,
         "value": round(ms_per_step, 5),
