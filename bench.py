@@ -100,14 +100,20 @@ def cpu_baseline(p: int, n: int, seconds: float, gpu_check):
             "res64": f"{r_cpu:016X}", "res64_equal_to_gpu": r_cpu == r_gpu}
 
 
-def batched_leg(seed: int, iters: int, chunk: int = 1000):
+def batched_leg(seed: int, iters: int, chunk: int = 1000, tune_loops: int = 0):
     """The batched uneven workload: 64 LL tests, exponents log-uniform in [1M, 20M] from
     splitmix64(seed), 10,000 iterations each; all advance together (one hipGraph launch per
     iteration) versus the same tests one at a time on the plain path.  Both timings include
-    each test's set-up (tables) and its final res64 readout."""
-    from cuda_amd import Batch, LucasPRP
+    each test's set-up (tables) and its final res64 readout; with tune_loops each length is
+    tuned once beforehand (untimed) and both sides run the tuned plans."""
+    from cuda_amd import Batch, LucasPRP, secured_log2_length
     from cuda_amd.workloads import batched_exponents
     ps = batched_exponents(seed)
+    if tune_loops:
+        for p in sorted({secured_log2_length(q): q for q in ps}.values()):
+            u = LucasPRP(p)
+            u.tune(tune_loops)
+            u.close()
     t0 = time.perf_counter()
     batch = Batch()
     ts = [LucasPRP(p) for p in ps]
@@ -250,6 +256,8 @@ def main() -> int:
                     help="requests of the served mix for the served leg (0: skip it)")
     ap.add_argument("--served-scale", type=float, default=0.1, help="iteration scale of the served leg")
     ap.add_argument("--no-checked", action="store_true", help="skip the Gerbicz-checked PRP leg")
+    ap.add_argument("--tune-loops", type=int, default=20,
+                    help="tune each length before use, this many loops per plan (0: default plans)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain copy-like kernel (int64 negate) over MIB MiB, three "
                          "times: the reference point that scales HBM byte counters")
@@ -279,6 +287,10 @@ def main() -> int:
     mode = "PRP" if args.prp else "LL"
     t = LucasPRP(p, prp=args.prp)
     n = t.n
+    tuned = None
+    if args.tune_loops:  # as the reference does at start: time every plan, keep the fastest
+        t.tune(args.tune_loops)
+        tuned = {"loops": args.tune_loops, "candidates_ms": {k: round(v, 4) for k, v in t.tune_results()}}
     plan = t.describe()
     log(f"bench: p={p} n={n} plan={plan}")
 
@@ -352,12 +364,14 @@ def main() -> int:
                                    "achieved": round(iter_bytes / (ms_per_step * 1e-3) / 1e9, 1),
                                    "frac": round(iter_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
         "valu_field_ops_per_cu_clock": {k: round(v, 3) for k, v in rates.items()},
+        "tuned": tuned,
     }
     if args.batch_iters:
-        line["batched"] = batched_leg(args.seed, args.batch_iters)
+        line["batched"] = batched_leg(args.seed, args.batch_iters, tune_loops=args.tune_loops)
     if args.served_requests:
         from cuda_amd.serve import served_mix
-        line["served"] = served_mix(requests=args.served_requests, iter_scale=args.served_scale, verify=6)
+        line["served"] = served_mix(requests=args.served_requests, iter_scale=args.served_scale, verify=6,
+                                    tune_loops=args.tune_loops)
     if not args.no_checked:
         line["checked"] = checked_leg()
     if not args.no_cpu:

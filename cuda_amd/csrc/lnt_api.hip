@@ -740,7 +740,9 @@ int lnt_tune(lnt_ctx *c, int loops) {
                     if (make_plan(c->p, n, tmp, o).empty()) opts.push_back(o);
                 }
     std::vector<Cand> cands;
-    const uint32_t warm = std::max(2, loops / 10);
+    // warm-up long enough to capture the candidate's iteration graph, so the timed loops
+    // replay it as the plain path does
+    const uint32_t warm = std::max<uint32_t>(GRAPH_ITERS, (uint32_t)loops / 10);
     for (const PlanOpts &o : opts) {
         lnt_ctx *t = nullptr;
         if (make_context(c->device, c->p, n, c->prp, &o, &t) != LNT_OK) continue;
@@ -779,9 +781,16 @@ int lnt_tune(lnt_ctx *c, int loops) {
     // every candidate must agree with the others before its time counts
     for (auto &cd : cands)
         if (cd.out != cands[0].out) return fail(c, LNT_DEVICE_ERROR, "tuner: candidates disagree on the residue");
+    // the fastest candidate; the context's current plan stays unless another is faster by
+    // more than 2% (timings of near-equal plans differ by noise, and a switch rebuilds tables)
     const Cand *best = &cands[0];
     for (auto &cd : cands)
         if (cd.ms < best->ms) best = &cd;
+    for (auto &cd : cands) {
+        Plan cp;
+        make_plan(c->p, n, cp, cd.o);
+        if (same_kernels(cp, c->pl) && best->ms > 0.98f * cd.ms) best = &cd;
+    }
     {
         std::lock_guard<std::mutex> lk(g_tuned_mu);
         g_tuned[{c->device, n}] = best->o;

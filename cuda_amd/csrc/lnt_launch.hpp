@@ -83,9 +83,10 @@ __global__ __launch_bounds__(BB * BC) __attribute__((amdgpu_waves_per_eu(BA == 1
     lnt_row3m_body<BA, BB, BC, MODE>(threadIdx.x, blockIdx.x, LView{smem, W}, P);
 }
 
-// ---- batched launches: many tests of one shape in one grid.  Workgroup b serves test
-// b / wgs (its parameters Ps[b / wgs], uniform per workgroup) as that test's workgroup
-// b % wgs.  The "-2" flag is per test (Ps[].sub2).  Each kernel takes a private copy of
+// ---- batched launches: many tests of one shape in one grid of wgs workgroups per slot.
+// Row kernels: workgroup b serves slot b / wgs (parameters Ps[b / wgs], uniform per
+// workgroup) as its workgroup b % wgs; column kernels interleave the slots (slot b % slots,
+// tile b / slots of the XCD-renumbered b).  The "-2" flag is per test (Ps[].sub2).  Each kernel takes a private copy of
 // its parameter block: read through a reference into global memory, every field would
 // have to be re-read after each global store (possible aliasing), which costs registers.
 template <int B1, int B2, int M>
@@ -111,40 +112,52 @@ template <int B1, int B2, int K, int M>
 __global__ __launch_bounds__(K * B2) void lnt_kb_col_fwd(const LntParams *__restrict__ Ps, u32 wgs) {
     constexpr u32 W = lnt_col_fwd_lds<B1, B2, K>();
     __shared__ u64 smem[W];
-    const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
-    const LntParams P = Ps[b / wgs];
+    // member-minor order: the tiles of each XCD's contiguous range cover every slot, so
+    // empty slots (and the members) spread over all XCDs while neighbouring tiles of one
+    // member stay in one XCD's range
+    const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x), slots = gridDim.x / wgs;
+    const LntParams P = Ps[b % slots];
     if (P.n == 0) return;  // an empty slot of the batch
-    lnt_col_fwd_body<B1, B2, K, M>(threadIdx.x, b % wgs, LView{smem, W}, P, P.sub2 != 0);
+    lnt_col_fwd_body<B1, B2, K, M>(threadIdx.x, b / slots, LView{smem, W}, P, P.sub2 != 0);
 }
 
 template <int B1, int B2, int K>
 __global__ __launch_bounds__(K * B2) void lnt_kb_col_inv(const LntParams *__restrict__ Ps, u32 wgs) {
     constexpr u32 W = B1 * (B2 + LNT_PAD) * K;
     __shared__ u64 smem[W];
-    const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
-    const LntParams P = Ps[b / wgs];
+    // member-minor order: the tiles of each XCD's contiguous range cover every slot, so
+    // empty slots (and the members) spread over all XCDs while neighbouring tiles of one
+    // member stay in one XCD's range
+    const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x), slots = gridDim.x / wgs;
+    const LntParams P = Ps[b % slots];
     if (P.n == 0) return;  // an empty slot of the batch
-    lnt_col_inv_body<B1, B2, K>(threadIdx.x, b % wgs, LView{smem, W}, P);
+    lnt_col_inv_body<B1, B2, K>(threadIdx.x, b / slots, LView{smem, W}, P);
 }
 
 template <int BA, int BB, int BC, int K, int M>
 __global__ __launch_bounds__(K * BB * BC) void lnt_kb_col3_fwd(const LntParams *__restrict__ Ps, u32 wgs) {
     constexpr u32 W = LntCol3<BA, BB, BC, K>::LDS;
     __shared__ u64 smem[W];
-    const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
-    const LntParams P = Ps[b / wgs];
+    // member-minor order: the tiles of each XCD's contiguous range cover every slot, so
+    // empty slots (and the members) spread over all XCDs while neighbouring tiles of one
+    // member stay in one XCD's range
+    const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x), slots = gridDim.x / wgs;
+    const LntParams P = Ps[b % slots];
     if (P.n == 0) return;  // an empty slot of the batch
-    lnt_col3_fwd_body<BA, BB, BC, K, M>(threadIdx.x, b % wgs, LView{smem, W}, P, P.sub2 != 0);
+    lnt_col3_fwd_body<BA, BB, BC, K, M>(threadIdx.x, b / slots, LView{smem, W}, P, P.sub2 != 0);
 }
 
 template <int BA, int BB, int BC, int K>
 __global__ __launch_bounds__(K * BB * BC) void lnt_kb_col3_inv(const LntParams *__restrict__ Ps, u32 wgs) {
     constexpr u32 W = LntCol3<BA, BB, BC, K>::LDS;
     __shared__ u64 smem[W];
-    const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x);
-    const LntParams P = Ps[b / wgs];
+    // member-minor order: the tiles of each XCD's contiguous range cover every slot, so
+    // empty slots (and the members) spread over all XCDs while neighbouring tiles of one
+    // member stay in one XCD's range
+    const u32 b = lnt_xcd_bid(blockIdx.x, gridDim.x), slots = gridDim.x / wgs;
+    const LntParams P = Ps[b % slots];
     if (P.n == 0) return;  // an empty slot of the batch
-    lnt_col3_inv_body<BA, BB, BC, K>(threadIdx.x, b % wgs, LView{smem, W}, P);
+    lnt_col3_inv_body<BA, BB, BC, K>(threadIdx.x, b / slots, LView{smem, W}, P);
 }
 
 template <int B1, int B2, int G>

@@ -14,11 +14,13 @@
 // counterpart there.  Residues, Gerbicz checkpoints and tables of admitted tests stay on
 // the device until the test finishes.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <vector>
@@ -26,9 +28,9 @@
 #include "../../include/lnt_api.h"
 
 // Default Gerbicz check interval in blocks.  A check costs one block of squarings plus
-// reading both checked residues back (measured at 2^23, block 1000: 336 ms against
-// 246 ms per 1000 iterations), so the overhead is about 1.4 / L: 50 blocks keep it near 3%.
-constexpr uint32_t kDefaultCheckBlocks = 50;
+// reading both checked residues back (measured at 2^23, block 1000: 312 ms against
+// 221 ms per 1000 iterations), so the overhead is about 1.4 / L: 80 blocks keep it under 2%.
+constexpr uint32_t kDefaultCheckBlocks = 80;
 
 namespace {
 
@@ -75,6 +77,8 @@ struct lnt_server {
     std::vector<Active> active;
     lnt_batch *batch[CLASSES] = {};
     uint64_t bytes_in_use = 0, steps = 0, admitted = 0, finished = 0, iterations_run = 0;
+    std::set<uint32_t> tuned;                // NTT lengths tuned (scheduler thread only)
+    std::atomic<uint64_t> tuned_count{0};  // its size, for lnt_server_stats
     double ms_per_iter[CLASSES] = {0.05, 0.05, 0.05};
     bool stop = false;
     std::string err;
@@ -154,6 +158,14 @@ struct lnt_server {
             a.cls = size_class(q.p);
             a.admit_s = now();
             int rc = lnt_create(cfg.device, q.p, nullptr, 0, q.prp ? LNT_MODE_PRP : LNT_MODE_LL, &a.ctx);
+            if (rc == LNT_OK && cfg.tune_loops && !tuned.count(lnt_length(a.ctx))) {
+                // first test of this length: time every plan (validated against each other)
+                // and keep the fastest for it and every later context of the length
+                tuned.insert(lnt_length(a.ctx));
+                tuned_count = tuned.size();
+                rc = lnt_tune(a.ctx, (int)cfg.tune_loops);
+                if (rc != LNT_OK) set_err(lnt_last_error(a.ctx));
+            }
             if (rc == LNT_OK && a.gz) {
                 // blocks per check: the configured value, or by default one check every
                 // kDefaultCheckBlocks blocks and never fewer than one per request
@@ -297,6 +309,7 @@ int lnt_server_stats(lnt_server *s, uint64_t *out, int cap) {
             if (lnt_batch_stats(s->batch[c], st, 3) == LNT_OK) caps += st[2];
         out[7] = caps;
     }
+    if (cap >= 9) out[8] = s->tuned_count.load();
     return LNT_OK;
 }
 

@@ -55,7 +55,7 @@ class ServerConfig(ctypes.Structure):
     """lnt_server_config"""
     _fields_ = [("device", ctypes.c_int), ("hbm_budget", ctypes.c_uint64), ("max_tests", ctypes.c_uint32),
                 ("step_ms", ctypes.c_float), ("gerbicz_block", ctypes.c_uint32),
-                ("gerbicz_check", ctypes.c_uint32)]
+                ("gerbicz_check", ctypes.c_uint32), ("tune_loops", ctypes.c_uint32)]
 
 
 class Result(ctypes.Structure):
@@ -405,9 +405,9 @@ class Server:
     library admits them by device footprint and runs every admitted test in one batch."""
 
     def __init__(self, hbm_budget: int = 200 << 30, max_tests: int = 0, step_ms: float = 5.0,
-                 gerbicz_block: int = 1000, gerbicz_check: int = 10, device: int = 0):
+                 gerbicz_block: int = 1000, gerbicz_check: int = 0, device: int = 0, tune_loops: int = 0):
         self._lib = load()
-        cfg = ServerConfig(device, hbm_budget, max_tests, step_ms, gerbicz_block, gerbicz_check)
+        cfg = ServerConfig(device, hbm_budget, max_tests, step_ms, gerbicz_block, gerbicz_check, tune_loops)
         h = ctypes.c_void_p()
         rc = self._lib.lnt_server_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != LNT_OK:
@@ -427,10 +427,10 @@ class Server:
         return [{f: getattr(buf[i], f) for f, _ in Result._fields_} for i in range(k)]
 
     def stats(self) -> dict:
-        out = (ctypes.c_uint64 * 8)()
-        self._lib.lnt_server_stats(self._h, out, 8)
+        out = (ctypes.c_uint64 * 9)()
+        self._lib.lnt_server_stats(self._h, out, 9)
         keys = ("queued", "running", "finished", "bytes_admitted", "steps", "iterations", "admitted",
-                "graph_captures")
+                "graph_captures", "lengths_tuned")
         return {k: int(v) for k, v in zip(keys, out)}
 
     def now(self) -> float:

@@ -45,9 +45,11 @@ def percentile(xs: List[float], q: float) -> float:
 
 
 def run_served(reqs: List[Request], s: float, budget: int, step_ms: float, block: int,
-               check: int) -> Dict[int, dict]:
-    """Submit each request at its arrival time; collect every result."""
-    srv = Server(hbm_budget=budget, step_ms=step_ms, gerbicz_block=block, gerbicz_check=check)
+               check: int, tune_loops: int = 0) -> Dict[int, dict]:
+    """Submit each request at its arrival time; collect every result.  With tune_loops the
+    server tunes the first test of each length (its time counts in the served figures)."""
+    srv = Server(hbm_budget=budget, step_ms=step_ms, gerbicz_block=block, gerbicz_check=check,
+                 tune_loops=tune_loops)
     t0 = time.perf_counter()
     base = srv.now()
     out: Dict[int, dict] = {}
@@ -72,17 +74,23 @@ def run_served(reqs: List[Request], s: float, budget: int, step_ms: float, block
     for res in out.values():  # times relative to the arrival clock
         for k in ("submit_s", "admit_s", "done_s"):
             res[k] -= base
-    out[-1] = {"peak_running": peak, "steps": st["steps"]}
+    out[-1] = {"peak_running": peak, "steps": st["steps"], "lengths_tuned": st["lengths_tuned"]}
     return out
 
 
-def plain_classes(reqs: List[Request], s: float, block: int, check: int) -> Dict[tuple, dict]:
-    """Plain-path cost per (log2 n, mode): set-up seconds and seconds per iteration."""
+def plain_classes(reqs: List[Request], s: float, block: int, check: int,
+                  tune_loops: int = 0) -> Dict[tuple, dict]:
+    """Plain-path cost per (log2 n, mode): set-up seconds and seconds per iteration (with
+    tune_loops, on the plan the tuner picks for the length, tuned once beforehand)."""
     classes: Dict[tuple, int] = {}
     for r in reqs:
         classes.setdefault((secured_log2_length(r.p), r.prp), r.p)
     out = {}
     for (ln, prp), p in sorted(classes.items()):
+        if tune_loops:
+            t = LucasPRP(p, prp=prp)
+            t.tune(tune_loops)
+            t.close()
         t0 = time.perf_counter()
         t = LucasPRP(p, prp=prp)
         if prp and block:
@@ -134,7 +142,7 @@ def run_alone(r: Request, s: float, block: int, check: int):
 
 def served_mix(requests: int = 2000, rate: float = 4.0, seed: int = 0, iter_scale: float = 1.0,
                budget_gb: float = 200.0, step_ms: float = 5.0, gerbicz_block: int = 1000,
-               gerbicz_check: int = 0, verify: int = 24) -> dict:
+               gerbicz_check: int = 0, verify: int = 24, tune_loops: int = 20) -> dict:
     """Serve the first `requests` of the mix at their arrival times and model the same stream
     one at a time on the plain path; returns the report (tests/hour, p50/p99 both ways)."""
     reqs = served_requests(seed, requests, rate)
@@ -144,12 +152,12 @@ def served_mix(requests: int = 2000, rate: float = 4.0, seed: int = 0, iter_scal
     log(f"serve: {len(reqs)} requests over {reqs[-1].arrival:.0f} s, iteration scale {iter_scale}, "
         f"Gerbicz block {block} x {check}")
 
-    classes = plain_classes(reqs, iter_scale, block, check)
+    classes = plain_classes(reqs, iter_scale, block, check, tune_loops)
     service = [plain_service(r, iter_scale, classes) for r in reqs]
     pdone = fifo(reqs, service)
     p_ttr = [d - r.arrival for d, r in zip(pdone, reqs)]
 
-    got = run_served(reqs, iter_scale, int(budget_gb * 2**30), step_ms, block, check)
+    got = run_served(reqs, iter_scale, int(budget_gb * 2**30), step_ms, block, check, tune_loops)
     meta = got.pop(-1)
     ttr = [got[r.index]["done_s"] - got[r.index]["submit_s"] for r in reqs]
     errors = sum(1 for v in got.values() if v["status"] != 0)
@@ -169,14 +177,15 @@ def served_mix(requests: int = 2000, rate: float = 4.0, seed: int = 0, iter_scal
         "workload": f"{len(reqs)} requests, Poisson rate {rate}/s (seed {seed}): 80% full LL "
                     f"p in [50k, 500k], 20% PRP chunks of 100,000 iterations p in [5M, 40M]; "
                     f"iterations x {iter_scale}; Gerbicz block {block} x {check}; HBM budget "
-                    f"{budget_gb:g} GB",
+                    f"{budget_gb:g} GB; each length tuned ({tune_loops} loops per candidate) "
+                    f"on first use" if tune_loops else f"{budget_gb:g} GB; default plans",
         "requests": len(reqs), "rate": rate, "iter_scale": iter_scale,
         "arrival_window_s": round(reqs[-1].arrival - reqs[0].arrival, 2),
         "served": {"tests_per_hour": round(len(reqs) / makespan * 3600, 1),
                    "makespan_s": round(makespan, 2),
                    "p50_s": round(percentile(ttr, 50), 3), "p99_s": round(percentile(ttr, 99), 3),
                    "mean_s": round(sum(ttr) / len(ttr), 3), "peak_running": meta["peak_running"],
-                   "steps": meta["steps"], "errors": errors,
+                   "steps": meta["steps"], "lengths_tuned": meta["lengths_tuned"], "errors": errors,
                    "ll_prime_found": sum(1 for v in got.values() if v["is_zero"] and not v["prp"])},
         "plain_one_at_a_time": {"tests_per_hour": round(len(reqs) / p_makespan * 3600, 1),
                                 "makespan_s": round(p_makespan, 2),
@@ -203,10 +212,12 @@ def main(argv=None) -> int:
     ap.add_argument("--gerbicz-block", type=int, default=1000)
     ap.add_argument("--gerbicz-check", type=int, default=0, help="blocks per check (0: up to 10 per chunk)")
     ap.add_argument("--verify", type=int, default=24, help="requests also run alone for real")
+    ap.add_argument("--tune-loops", type=int, default=20,
+                    help="tune each NTT length on first use with this many loops per plan (0: default plans)")
     ap.add_argument("--json", default="")
     a = ap.parse_args(argv)
     out = served_mix(a.requests, a.rate, a.seed, a.iter_scale, a.budget_gb, a.step_ms, a.gerbicz_block,
-                     a.gerbicz_check, a.verify)
+                     a.gerbicz_check, a.verify, a.tune_loops)
     line = json.dumps(out)
     print(line, flush=True)
     if a.json:

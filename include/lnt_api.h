@@ -67,7 +67,8 @@ int lnt_res64(lnt_ctx *ctx, uint64_t *out);
 int lnt_sync(lnt_ctx *ctx, float *elapsed_ms);
 
 /* Time `loops` iterations on scratch state for every valid engine configuration of this
- * length, validate each against the current configuration first, keep the fastest.
+ * length, validate each against the current configuration first, keep the fastest (the
+ * current configuration unless another is faster by more than 2%).
  * The residue is preserved.  Replaces Process::GetBestPerformers / EvaluatePerformance
  * (process.cu:45-149), which timed candidates without checking their results. */
 int lnt_tune(lnt_ctx *ctx, int loops);
@@ -197,7 +198,11 @@ typedef struct lnt_server_config {
     uint32_t max_tests;      /* admitted tests at most (0: no limit) */
     float step_ms;           /* target GPU time of one scheduling step (0: 5 ms) */
     uint32_t gerbicz_block;  /* PRP tests: Gerbicz block (0: unchecked) */
-    uint32_t gerbicz_check;  /* blocks per check (0: 50, or the request's block count if fewer) */
+    uint32_t gerbicz_check;  /* blocks per check (0: 80, or the request's block count if fewer) */
+    uint32_t tune_loops;     /* > 0: the first test of each NTT length is tuned (lnt_tune with this
+                              * many loops per candidate) before it joins, and later tests of that
+                              * length get the tuned plan; 0: default plans (the reference re-tunes
+                              * at every start, process.cu:45-127) */
 } lnt_server_config;
 typedef struct lnt_result {
     uint64_t tag;            /* the caller's id given to lnt_server_submit */
@@ -219,7 +224,8 @@ int lnt_server_submit(lnt_server *server, uint32_t p, int prp, uint64_t iteratio
 int lnt_server_poll(lnt_server *server, lnt_result *out, int cap, int wait_ms);
 /* out[0] queued, [1] admitted and running, [2] finished, [3] device bytes admitted,
  * [4] scheduling steps, [5] iterations per member summed over steps, [6] admitted,
- * [7] (cap >= 8) iteration graphs captured by the server's batches.  cap >= 7. */
+ * [7] (cap >= 8) iteration graphs captured by the server's batches, [8] (cap >= 9) NTT
+ * lengths tuned.  cap >= 7. */
 int lnt_server_stats(lnt_server *server, uint64_t *out, int cap);
 double lnt_server_now(lnt_server *server);
 /* Stop the scheduler; tests still running end with status LNT_INVALID_ARGUMENT. */

@@ -84,6 +84,24 @@ def test_budget_limits_admission():
         assert got[i]["res64"] == alone(p, False, 400)[0]
 
 
+def test_server_tunes_each_length_once():
+    """With tune_loops the first test of each NTT length is tuned before it joins; every
+    result still equals the test alone (res64 pinned by the plain path)."""
+    reqs = [(200003, False, 1500), (200017, False, 1200), (1000003, False, 800), (1000033, True, 900),
+            (86243, False, 3000)]
+    srv = Server(hbm_budget=8 << 30, step_ms=2.0, gerbicz_block=0, tune_loops=5)
+    for i, (p, prp, k) in enumerate(reqs):
+        srv.submit(p, prp, k, i)
+    got = collect(srv, len(reqs))
+    st = srv.stats()
+    srv.close()
+    lengths = {cuda_amd.secured_log2_length(p) for p, _, _ in reqs}
+    assert st["lengths_tuned"] == len(lengths)
+    for i, (p, prp, k) in enumerate(reqs):
+        assert got[i]["status"] == 0, (p, got[i])
+        assert got[i]["res64"] == alone(p, prp, k)[0], (p, prp, k)
+
+
 def test_served_mix_sample_matches_gmp():
     """The served mix (cuda_amd.workloads.served_requests, seed 0): its GMP-pinned sample of
     full LL tests (p in [50k, 500k], two of them at length 2^11) and PRP chunks (p in
