@@ -521,10 +521,13 @@ LNT_KFN void lnt_row_body(int tid, int bid, LView lds, const LntParams &P) {
 // reverse.  Within the LDS row of k2 a word (r, a) sits at r * (B + 1) + a, so every
 // phase reads and writes conflict-free strides.  Transform-domain slot for MODE 1/2:
 // base + k2 * B^2 + kb * B + i (thread-contiguous).
-template <int B, int G, int MODE = LNT_SQUARE>
+// LDS layout: word (k2, a) of a row at k2 * RS + a with runs of B words, unpadded by default:
+// the phase-3 runs are then 16-byte aligned and pair into 128-bit LDS accesses (at n = 2^22
+// the unpadded 4096-word row ran 53 against 55 us with one pad word per run).
+template <int B, int G, int MODE = LNT_SQUARE, int PAD = 0>
 LNT_KFN void lnt_row3_body(int tid, int bid, LView lds, const LntParams &P) {
     constexpr int T = B * B;        // threads per row
-    constexpr int RS = B * (B + 1); // LDS words per k2 row
+    constexpr int RS = B * (B + PAD); // LDS words per k2 row
     constexpr int LINE = B * RS;    // LDS words per row
     const int t = tid % T, g = tid / T;
     const u64 base = (u64)P.C * ((u64)bid * G + g);
@@ -548,21 +551,21 @@ LNT_KFN void lnt_row3_body(int tid, int bid, LView lds, const LntParams &P) {
 #pragma unroll
     for (int rho = 1; rho < B; rho++) a[rho] = gl_mul(a[rho], P.twr[(u64)rho * T + t]);
 #pragma unroll
-    for (int rho = 0; rho < B; rho++) lds[lb + (u32)(brev_c<B>(rho) * RS + (t / B) * (B + 1) + t % B)] = a[rho];
+    for (int rho = 0; rho < B; rho++) lds[lb + (u32)(brev_c<B>(rho) * RS + (t / B) * (B + PAD) + t % B)] = a[rho];
     LNT_SYNC();
     // phase 2: thread (k2 = hi, a = lo) over b
 #pragma unroll
-    for (int b = 0; b < B; b++) a[b] = lds[lb + (u32)(hi * RS + b * (B + 1) + lo)];
+    for (int b = 0; b < B; b++) a[b] = lds[lb + (u32)(hi * RS + b * (B + PAD) + lo)];
     // (this thread rewrites exactly the slots it just read: no barrier needed)
     dft_fwd<B>(a);
 #pragma unroll
     for (int rho = 1; rho < B; rho++) a[rho] = gl_mul(a[rho], P.twr2[(u64)rho * B + lo]);
 #pragma unroll
-    for (int rho = 0; rho < B; rho++) lds[lb + (u32)(hi * RS + brev_c<B>(rho) * (B + 1) + lo)] = a[rho];
+    for (int rho = 0; rho < B; rho++) lds[lb + (u32)(hi * RS + brev_c<B>(rho) * (B + PAD) + lo)] = a[rho];
     LNT_SYNC();
     // phase 3: thread (k2 = hi, kb = lo) over a
 #pragma unroll
-    for (int i = 0; i < B; i++) a[i] = lds[lb + (u32)(hi * RS + lo * (B + 1) + i)];
+    for (int i = 0; i < B; i++) a[i] = lds[lb + (u32)(hi * RS + lo * (B + PAD) + i)];
     dft_fwd<B>(a);
     if (MODE == LNT_FORWARD) {
 #pragma unroll
@@ -578,22 +581,22 @@ LNT_KFN void lnt_row3_body(int tid, int bid, LView lds, const LntParams &P) {
     }
     dft_inv<B>(a);
 #pragma unroll
-    for (int i = 0; i < B; i++) lds[lb + (u32)(hi * RS + lo * (B + 1) + i)] = a[i];
+    for (int i = 0; i < B; i++) lds[lb + (u32)(hi * RS + lo * (B + PAD) + i)] = a[i];
     LNT_SYNC();
     // phase 2 inverse: thread (k2, a): registers rho hold kb = brev(rho)
 #pragma unroll
     for (int rho = 0; rho < B; rho++) {
-        u64 v = lds[lb + (u32)(hi * RS + brev_c<B>(rho) * (B + 1) + lo)];
+        u64 v = lds[lb + (u32)(hi * RS + brev_c<B>(rho) * (B + PAD) + lo)];
         a[rho] = rho ? gl_mul(v, P.itwr2[(u64)rho * B + lo]) : v;
     }
     dft_inv<B>(a);
 #pragma unroll
-    for (int b = 0; b < B; b++) lds[lb + (u32)(hi * RS + b * (B + 1) + lo)] = a[b];
+    for (int b = 0; b < B; b++) lds[lb + (u32)(hi * RS + b * (B + PAD) + lo)] = a[b];
     LNT_SYNC();
     // phase 1 inverse: thread i1 = t, registers rho hold k2 = brev(rho)
 #pragma unroll
     for (int rho = 0; rho < B; rho++) {
-        u64 v = lds[lb + (u32)(brev_c<B>(rho) * RS + (t / B) * (B + 1) + t % B)];
+        u64 v = lds[lb + (u32)(brev_c<B>(rho) * RS + (t / B) * (B + PAD) + t % B)];
         a[rho] = rho ? gl_mul(v, P.itwr[(u64)rho * T + t]) : v;
     }
     dft_inv<B>(a);

@@ -66,9 +66,11 @@ __global__ __launch_bounds__(G * B2) void lnt_k_row(LntParams P) {
     lnt_row_body<B1, B2, G, MODE>(threadIdx.x, blockIdx.x, LView{smem, W}, P);
 }
 
+// 256-thread rows of 16^3 (or 4 rows of 8^3) words in 32 KiB of LDS; 128 registers per
+// lane keep 4 waves per SIMD
 template <int B, int G, int MODE>
-__global__ __launch_bounds__(G * B * B) void lnt_k_row3(LntParams P) {
-    constexpr u32 W = G * B * B * (B + 1);
+__global__ __launch_bounds__(G * B * B) __attribute__((amdgpu_waves_per_eu(4))) void lnt_k_row3(LntParams P) {
+    constexpr u32 W = G * B * B * B;
     __shared__ u64 smem[W];
     lnt_row3_body<B, G, MODE>(threadIdx.x, blockIdx.x, LView{smem, W}, P);
 }
@@ -170,8 +172,9 @@ __global__ __launch_bounds__(G * B2) void lnt_kb_row(const LntParams *__restrict
 }
 
 template <int B, int G>
-__global__ __launch_bounds__(G * B * B) void lnt_kb_row3(const LntParams *__restrict__ Ps, u32 wgs) {
-    constexpr u32 W = G * B * B * (B + 1);
+__global__ __launch_bounds__(G * B * B) __attribute__((amdgpu_waves_per_eu(4))) void lnt_kb_row3(
+    const LntParams *__restrict__ Ps, u32 wgs) {
+    constexpr u32 W = G * B * B * B;
     __shared__ u64 smem[W];
     const LntParams P = Ps[blockIdx.x / wgs];
     if (P.n == 0) return;  // an empty slot of the batch

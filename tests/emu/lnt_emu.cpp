@@ -239,7 +239,7 @@ static void col3_inv(Emu *e, const LntParams &P) {
 template <int B, int G>
 static void row3_sq(Emu *e, const LntParams &P, int mode) {
     const Plan &pl = e->pl;
-    uint32_t lds = G * B * B * (B + 1);
+    uint32_t lds = G * B * B * B;
     for (uint32_t b = 0; b < pl.R / G; b++)
         run_wg(G * B * B, lds, [&](int tid, LView L) {
             if (mode == LNT_FORWARD) lnt_row3_body<B, G, LNT_FORWARD>(tid, b, L, P);
