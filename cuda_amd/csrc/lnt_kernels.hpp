@@ -58,11 +58,6 @@ struct LntParams {
     u32 sub2;                // batched launches: this test's LL "-2" is due at the next carry
 };
 
-// e_g = ceil(p g / n) for 0 <= g <= n (bit offset of digit g).
-GL_HD u32 lnt_digit_offset(u32 p, u32 log_n, u64 g) {
-    return (u32)(((u64)p * g + ((1ull << log_n) - 1)) >> log_n);
-}
-
 // Weight exponents on the large path.  Digit i = c + C r has weight B^e_i, e_i = (-p i) mod n
 // (B^n = 2), and e_i = e_c + f_r - n [e_c + f_r >= n] with e_c = (-p c) mod n and
 // f_r = (-p C r) mod n.  So w_i = B^e_c * B^f_r * 2^-[wrap]: the column factor B^e_c is
@@ -89,30 +84,24 @@ template <int M, int K>
 GL_HD void lnt_carry_run(int64_t (&v)[M + K], u32 p, u32 log_n, u64 g0, u32 levels,
                          bool sub2) {
     const u64 n = 1ull << log_n;
+    // widths: with R_g = (p g + n - 1) mod n, the width of digit g is
+    // floor((R_g + p) / n) = p / n + [R_g + (p mod n) >= n] and R_(g+1) = (R_g + p) mod n;
+    // R_n = R_0, so the recurrence runs across the wrap from digit n-1 to digit 0
+    const u32 nm = (u32)n - 1, pm = p & nm, wmin = p >> log_n;
+    u32 R = ((u32)p * (u32)((g0 + n - M) & (n - 1)) + nm) & nm;
     int w[M + K];
-    u64 g = (g0 + n - M) & (n - 1);
-    u32 e = lnt_digit_offset(p, log_n, g);
 #pragma unroll
     for (int j = 0; j < M + K; j++) {
-        u64 g1 = g + 1;
-        u32 e1 = lnt_digit_offset(p, log_n, g1);
-        w[j] = (int)(e1 - e);
-        e = e1;
-        g = g1;
-        if (g == n) {
-            g = 0;
-            e = 0;
-        }
+        const u32 s = R + pm;
+        w[j] = (int)(wmin + (s >> log_n));
+        R = s & nm;
     }
-    // LL "-2" on digit 0: in its own run, or in the halo of a run starting at g0 <= M
-    // (every window that holds digit 0 must see the same value)
-    if (sub2) {
-        if (g0 == 0) v[M] -= 2;
-        else if (g0 <= (u64)M) {
+    // LL "-2" on digit 0, at window position M - g0 when g0 <= M: in the run itself
+    // (g0 = 0) or in the halo of a run just after it (every window that holds digit 0 must
+    // see the same value); a select per position, no indexing by a run-time value
+    if (sub2 && g0 <= (u64)M) {
 #pragma unroll
-            for (int j = 0; j < M; j++)
-                if (j == M - (int)g0) v[j] -= 2;
-        }
+        for (int j = 0; j <= M; j++) v[j] -= (g0 + (u64)j == (u64)M) ? 2 : 0;
     }
     for (u32 l = 0; l < levels; l++) {
         int64_t hprev = 0;
