@@ -319,8 +319,9 @@ LNT_KFN void lnt_col3_fwd_body(int tid, int bid, LView lds, const LntParams &P, 
         const int r = tid + NT * q;
         if (r >= RR) break;
         // streamed in pieces of PK digits, each piece's halo being the M pre-carry values
-        // before it (blocks of 1024 threads keep the run within their 128 registers)
-        constexpr int PK = (NT >= 1024 && K > 8) ? 8 : K;
+        // before it: the whole 16-digit run in one piece where it fits the 128 registers of
+        // a 1024-thread block (BA = 16, M = 4), pieces of 8 where one piece would spill
+        constexpr int PK = (NT >= 1024 && K > 8 && (M > 4 || BA < 16)) ? 8 : K;
         const u64 g0 = C * (u64)r + c0;
         int64_t halo[M];
 #pragma unroll
