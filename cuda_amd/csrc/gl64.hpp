@@ -110,6 +110,18 @@ GL_HD u64 gl_sqr(u64 a) {
 #endif
 }
 
+// (db - 2^31) w mod P for a biased small integer db < 2^32, given wb = -2^31 w mod P:
+// the 96-bit sum db w + wb from two 32x64 multiply-adds, then hi:lo = lo + hi (2^32 - 1)
+// with hi < 2^32.  Half the work of gl_mul for the carried digits (|d| < 2^31) that the
+// weights multiply.  Canonical output for canonical w, wb.
+GL_HD u64 gl_mul_biased(u32 db, u64 w, u64 wb) {
+    const u64 m0 = (u64)db * (u32)w + (u32)wb;                            // < 2^64
+    const u64 m1 = (u64)db * (u32)(w >> 32) + (m0 >> 32) + (wb >> 32);    // < 2^64
+    const u64 lo = (m1 << 32) | (u32)m0;
+    const u32 x2 = (u32)(m1 >> 32);
+    return gl_add(lo, ((u64)x2 << 32) - x2);  // x2 (2^32 - 1) <= P - 2^32
+}
+
 // x * 2^s mod P for 0 <= s < 96 (s is a compile-time constant after inlining in every
 // caller, so the branches fold away).  Canonical output for any 64-bit x.
 GL_HD u64 gl_mul_pow2_lt96(u64 x, int s) {

@@ -100,7 +100,7 @@ static int build_tables(lnt_ctx *c) {
     HIPCHK(c, hipMalloc(&c->x, nb), "hipMalloc (x)");
     HIPCHK(c, hipMalloc(&c->y, nb), "hipMalloc (work buffer)");
     HIPCHK(c, hipMalloc(&c->w, (size_t)z.W * 8), "hipMalloc (weights)");
-    HIPCHK(c, hipMalloc(&c->uw, (size_t)z.W * 8), "hipMalloc (unweights)");
+    HIPCHK(c, hipMalloc(&c->uw, (size_t)z.Wu * 8), "hipMalloc (unweights)");
     HIPCHK(c, hipMalloc(&c->twr, (size_t)z.Lr * 8), "hipMalloc (row twiddles)");
     HIPCHK(c, hipMalloc(&c->itwr, (size_t)z.Lr * 8), "hipMalloc (row twiddles)");
     if (z.large) {
@@ -1353,7 +1353,7 @@ bool static_layout(const Plan &pl, u64 *base, TablePtrs &t) {
     };
     t.y = take(z.n);
     t.w = take(z.W);
-    t.uw = take(z.W);
+    t.uw = take(z.Wu);
     t.twr = take(z.Lr);
     t.itwr = take(z.Lr);
     if (z.large) {

@@ -63,6 +63,7 @@ struct Plan {
     uint32_t cBB = 0, cBC = 0;  // 3-phase columns: second and third radix
     uint32_t M = 0;         // carry halo (compile-time), >= levels
     uint32_t levels = 0;    // carry levels
+    bool bias = false;      // large path: every carried digit |d| < 2^31, whatever the input
 };
 
 // Two plans run the same kernel instantiations over the same table layouts (only the
@@ -113,6 +114,19 @@ inline uint32_t carry_levels(uint32_t p, uint32_t n) {
         }
     }
     return 0;
+}
+
+// After `levels` carry levels, is every digit below 2^31 in magnitude for any input value
+// |v| < 2^63 (a field element read as a signed integer, corrupted ones included)?  Each level
+// maps a bound V to 2^(wmax-1) + floor(V / 2^wmin) + 1.  Then d + 2^31 fits 32 bits and the
+// weight multiply can take the biased digit (gl_mul_biased).
+inline bool digits_fit_31(uint32_t p, uint32_t n, uint32_t levels) {
+    const uint32_t L = ilog2(n);
+    const uint32_t wmin = p >> L, wmax = (p + n - 1) >> L;
+    if (wmin == 0 || wmax > 32) return false;
+    uint64_t V = 1ull << 63;
+    for (uint32_t l = 0; l < levels; l++) V = (1ull << (wmax - 1)) + (V >> wmin) + 1;
+    return V < (1ull << 31);
 }
 
 // Row/column split of a large residue: log2 C and log2 R.
@@ -197,6 +211,7 @@ inline std::string make_plan_fields(uint32_t p, uint64_t n, Plan &pl, PlanOpts o
     if ((uint64_t)p / n > 31) return "NTT length too small for this exponent (digits wider than 31 bits)";
     pl.levels = carry_levels(p, (uint32_t)n);
     if (pl.levels == 0) return "NTT length too small for this exponent (convolution would overflow)";
+    pl.bias = digits_fit_31(p, (uint32_t)n, pl.levels);
     pl.M = pl.levels <= 4 ? 4 : 8;
     if (pl.log_n <= 12) {
         pl.small = true;

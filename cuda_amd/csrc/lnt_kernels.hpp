@@ -45,7 +45,8 @@ struct LntParams {
     GView<u64> x;            // n words: residue state (pre-carry values between iterations)
     GView<u64> y;            // n words: transform-domain work buffer
     GView<const u64> w, uw;  // IBDWT weights, unweights * n^-1: n words in digit order (one-workgroup
-                             // path); 2R row factors indexed by lnt_wslot (large path)
+                             // path); large path: row factors indexed by lnt_wslot, the weights
+                             // as pairs (w, -2^31 w) at 2 * slot, the unweights at slot
     GView<const u64> tw4, itw4;  // 4-step twiddles (large path), natural slots, times the column
                                  // weight factor (unweight factor * n^-1)
     GView<const u64> twr, itwr;  // row-line twiddles (C words)
@@ -56,6 +57,7 @@ struct LntParams {
     u32 p, n, log_n, C, R;
     u32 levels;              // carry levels m
     u32 sub2;                // batched launches: this test's LL "-2" is due at the next carry
+    u32 bias;                // large path: carried digits go to the weight multiply as d + 2^31
 };
 
 // Weight exponents on the large path.  Digit i = c + C r has weight B^e_i, e_i = (-p i) mod n
@@ -164,7 +166,7 @@ LNT_KFN void lnt_col_fwd_body(int tid, int bid, LView lds, const LntParams &P, b
     }
     if (PF) {
 #pragma unroll
-        for (int j = 0; j < B1; j++) wv[PF ? j : 0] = P.w[lnt_wslot(ec, mC, P.n, (u32)(t + B2 * j))];
+        for (int j = 0; j < B1; j++) wv[PF ? j : 0] = P.w[2 * lnt_wslot(ec, mC, P.n, (u32)(t + B2 * j))];
     }
 #pragma unroll
     for (int j = 0; j < B1; j++) lds[(u32)((t + B2 * j) * KS + cl)] = a[j];
@@ -191,7 +193,7 @@ LNT_KFN void lnt_col_fwd_body(int tid, int bid, LView lds, const LntParams &P, b
 #pragma unroll
     for (int j = 0; j < B1; j++)
         a[j] = gl_mul(lds[(u32)((t + B2 * j) * KS + cl)],
-                      PF ? wv[PF ? j : 0] : P.w[lnt_wslot(ec, mC, P.n, (u32)(t + B2 * j))]);
+                      PF ? wv[PF ? j : 0] : P.w[2 * lnt_wslot(ec, mC, P.n, (u32)(t + B2 * j))]);
     dft_fwd<B1>(a);
     if (B2 > 1) {
 #pragma unroll
@@ -293,7 +295,7 @@ LNT_KFN void lnt_col3_fwd_body(int tid, int bid, LView lds, const LntParams &P, 
     //    digits of this thread's carry rows, the weights (prefetched into registers only
     //    in blocks of <= 512 threads, whose register budget allows it)
     constexpr bool PW = NT <= 512;
-    u64 a[BA], wv[PW ? BA : 1], h[HP ? RT : 1][M];
+    u64 a[BA], wv[PW ? BA : 1], wb[PW ? BA : 1], h[HP ? RT : 1][M];
 #pragma unroll
     for (int j = 0; j < BA; j++) a[j] = P.x[c + C * (u64)(t + A * j)];
     if (HP) {
@@ -307,7 +309,11 @@ LNT_KFN void lnt_col3_fwd_body(int tid, int bid, LView lds, const LntParams &P, 
     }
     if (PW) {
 #pragma unroll
-        for (int j = 0; j < BA; j++) wv[PW ? j : 0] = P.w[lnt_wslot(ec, mC, P.n, (u32)(t + A * j))];
+        for (int j = 0; j < BA; j++) {
+            const u32 sl = lnt_wslot(ec, mC, P.n, (u32)(t + A * j));
+            wv[PW ? j : 0] = P.w[2 * sl];
+            wb[PW ? j : 0] = P.w[2 * sl + 1];
+        }
     }
 #pragma unroll
     for (int j = 0; j < BA; j++) lds[(u32)((t + A * j) * KS + cl)] = a[j];
@@ -336,17 +342,29 @@ LNT_KFN void lnt_col3_fwd_body(int tid, int bid, LView lds, const LntParams &P, 
 #pragma unroll
             for (int j = 0; j < M; j++) halo[j] = v[PK + j];
             lnt_carry_run<M, PK>(v, P.p, P.log_n, g0 + s * PK, P.levels, sub2);
+            // the digit as d + 2^31 for the biased weight multiply, else field-encoded
 #pragma unroll
-            for (int j = 0; j < PK; j++) lds[(u32)(r * KS + s * PK + j)] = gl_from_i64(v[M + j]);
+            for (int j = 0; j < PK; j++)
+                lds[(u32)(r * KS + s * PK + j)] =
+                    P.bias ? (u64)(u32)(v[M + j] + (int64_t(1) << 31)) : gl_from_i64(v[M + j]);
         }
     }
     LNT_SYNC();
 
     // 2. phase 1: weight, BA-point DFT, twiddle
+    if (P.bias) {
 #pragma unroll
-    for (int j = 0; j < BA; j++)
-        a[j] = gl_mul(lds[(u32)((t + A * j) * KS + cl)],
-                      PW ? wv[PW ? j : 0] : P.w[lnt_wslot(ec, mC, P.n, (u32)(t + A * j))]);
+        for (int j = 0; j < BA; j++) {
+            const u32 sl = PW ? 0 : lnt_wslot(ec, mC, P.n, (u32)(t + A * j));
+            a[j] = gl_mul_biased((u32)lds[(u32)((t + A * j) * KS + cl)], PW ? wv[PW ? j : 0] : P.w[2 * sl],
+                                 PW ? wb[PW ? j : 0] : P.w[2 * sl + 1]);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < BA; j++)
+            a[j] = gl_mul(lds[(u32)((t + A * j) * KS + cl)],
+                          PW ? wv[PW ? j : 0] : P.w[2 * lnt_wslot(ec, mC, P.n, (u32)(t + A * j))]);
+    }
     dft_fwd<BA>(a);
 #pragma unroll
     for (int rho = 1; rho < BA; rho++) a[rho] = gl_mul(a[rho], P.twc[(u64)rho * A + t]);
@@ -966,6 +984,12 @@ GL_HD u64 lnt_tw4_3(u64 omega_n, u32 C, u32 BA, u32 BB, u32 BC, u64 idx) {
 GL_HD u64 lnt_row_factor(u64 B, u64 m, u32 p, u32 n, u32 C, u32 slot) {
     const u64 v = gl_pow(B, lnt_wexp(p, n, (u64)C * (slot >> 1)));
     return (slot & 1) ? gl_mul(v, m) : v;
+}
+// Word i of the large-path weight table: the pair (w, -2^31 w) of slot i / 2
+// (gl_mul_biased takes both).
+GL_HD u64 lnt_weight_pair(u64 Bw, u32 p, u32 n, u32 C, u32 i) {
+    const u64 w = lnt_row_factor(Bw, (GL_P + 1) / 2, p, n, C, i >> 1);
+    return (i & 1) ? gl_mul(w, GL_P - (1ull << 31)) : w;
 }
 // Column factor B^e_c times s (s = 1 for weights, n^-1 for unweights), c = idx mod C.
 GL_HD u64 lnt_col_factor(u64 B, u64 s, u32 p, u32 n, u32 C, u64 idx) {

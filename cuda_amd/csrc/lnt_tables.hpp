@@ -25,11 +25,12 @@ static __global__ void lnt_k_weights(u64 Bw, u64 Binv, u64 ninv, u32 p, u32 n, u
     uw[i] = gl_mul(gl_pow(Binv, (u64)(n - r)), ninv);
 }
 
+// large path: L = 2R row slots; weights as (w, -2^31 w) pairs (2L words), unweights (L words)
 static __global__ void lnt_k_row_weights(u64 Bw, u64 Binv, u32 p, u32 n, u32 C, u32 L, u64 *w, u64 *uw) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= L) return;
-    w[i] = lnt_row_factor(Bw, (GL_P + 1) / 2, p, n, C, i);
-    uw[i] = lnt_row_factor(Binv, 2, p, n, C, i);
+    if (i >= 2 * L) return;
+    w[i] = lnt_weight_pair(Bw, p, n, C, i);
+    if (i < L) uw[i] = lnt_row_factor(Binv, 2, p, n, C, i);
 }
 
 static __global__ void lnt_k_line_tw(u64 om, u32 B1, u32 B2, u32 L, u64 *tw) {
@@ -59,7 +60,7 @@ static GView<T> view(T *p, u64 n) {
 // Word counts of the tables of a plan (0 where the plan has no such table).
 struct TableSizes {
     u32 n, Lr, L2, R, A;  // residue, row line, second-level row line, column line, second-level column line
-    u32 W;                // each weight table: n digit weights (one workgroup) or 2R row factors
+    u32 W, Wu;            // weight / unweight table words: n each (one workgroup) or 4R / 2R
     bool large;
 };
 static TableSizes table_sizes(const Plan &pl) {
@@ -70,7 +71,8 @@ static TableSizes table_sizes(const Plan &pl) {
     t.L2 = pl.row3m ? pl.rBB * pl.rBC : pl.row3 ? pl.row3 * pl.row3 : 0;
     t.R = pl.small ? 0 : pl.R;
     t.A = (!pl.small && pl.col3) ? pl.R / pl.col3 : 0;
-    t.W = pl.small ? pl.n : 2 * pl.R;
+    t.W = pl.small ? pl.n : 4 * pl.R;
+    t.Wu = pl.small ? pl.n : 2 * pl.R;
     return t;
 }
 
@@ -88,8 +90,8 @@ static void fill_tables(const Plan &pl, u32 p, const TablePtrs &t, hipStream_t s
     const u64 Bw = gl_pow(7, (GL_P - 1) / 192 / n * 5);
     const u64 ninv = gl_inv(n), Binv = gl_inv(Bw);
     if (z.large)
-        hipLaunchKernelGGL(lnt_k_row_weights, dim3((z.W + 255) / 256), dim3(256), 0, s, Bw, Binv, p, n, pl.C, z.W, t.w,
-                           t.uw);
+        hipLaunchKernelGGL(lnt_k_row_weights, dim3((z.W + 255) / 256), dim3(256), 0, s, Bw, Binv, p, n, pl.C, z.Wu,
+                           t.w, t.uw);
     else
         hipLaunchKernelGGL(lnt_k_weights, dim3((n + 255) / 256), dim3(256), 0, s, Bw, Binv, ninv, p, n, t.w, t.uw);
     const u32 Lr = z.Lr;
@@ -145,7 +147,7 @@ static LntParams params_of(const Plan &pl, u32 p, u64 *x, const TablePtrs &t) {
     P.x = view<u64>(x, n);
     P.y = view<u64>(t.y, n);
     P.w = view<const u64>(t.w, z.W);
-    P.uw = view<const u64>(t.uw, z.W);
+    P.uw = view<const u64>(t.uw, z.Wu);
     P.tw4 = view<const u64>(t.tw4, z.large ? n : 0);
     P.itw4 = view<const u64>(t.itw4, z.large ? n : 0);
     P.twr = view<const u64>(t.twr, z.Lr);
@@ -162,6 +164,7 @@ static LntParams params_of(const Plan &pl, u32 p, u64 *x, const TablePtrs &t) {
     P.C = pl.C;
     P.R = pl.R;
     P.levels = pl.levels;
+    P.bias = (!pl.small && pl.bias) ? 1u : 0u;
     return P;
 }
 

@@ -70,13 +70,11 @@ extern "C" void *emu_create3(uint32_t p, uint32_t n, int prp, uint32_t lc, uint3
             e->w[i] = lnt_weight(Bw, p, n, i);
             e->uw[i] = gl_mul(gl_inv(e->w[i]), ninv);
         }
-    } else {  // row factors; the column factors go into tw4 / itw4 below
-        e->w.resize(2 * pl.R);
+    } else {  // row factors (weights as pairs); the column factors go into tw4 / itw4 below
+        e->w.resize(4 * pl.R);
         e->uw.resize(2 * pl.R);
-        for (uint32_t i = 0; i < 2 * pl.R; i++) {
-            e->w[i] = lnt_row_factor(Bw, (GL_P + 1) / 2, p, n, pl.C, i);
-            e->uw[i] = lnt_row_factor(Binv, 2, p, n, pl.C, i);
-        }
+        for (uint32_t i = 0; i < 4 * pl.R; i++) e->w[i] = lnt_weight_pair(Bw, p, n, pl.C, i);
+        for (uint32_t i = 0; i < 2 * pl.R; i++) e->uw[i] = lnt_row_factor(Binv, 2, p, n, pl.C, i);
     }
     widths(p, n, e->wid);
     if (pl.row3m) {
@@ -153,6 +151,7 @@ extern "C" void *emu_create3(uint32_t p, uint32_t n, int prp, uint32_t lc, uint3
     P.C = pl.C;
     P.R = pl.R;
     P.levels = pl.levels;
+    P.bias = (!pl.small && pl.bias) ? 1u : 0u;
     std::vector<uint64_t> d(n, 0);
     d[0] = prp ? 3 : 4;
     to_device_digits(e->wid, d.data(), e->x.data());
@@ -396,6 +395,11 @@ extern "C" void emu_set_digits(void *h, const uint64_t *d) {
 extern "C" uint32_t emu_levels(void *h) { return ((Emu *)h)->pl.levels; }
 extern "C" uint32_t emu_secured(uint32_t p) { return secured_log2_length(p); }
 extern "C" uint32_t emu_carry_levels(uint32_t p, uint32_t n) { return carry_levels(p, n); }
+extern "C" uint64_t emu_gl_mul_biased(uint32_t db, uint64_t w, uint64_t wb) { return gl_mul_biased(db, w, wb); }
+extern "C" int emu_digits_fit_31(uint32_t p, uint32_t n) {
+    const uint32_t levels = carry_levels(p, n);
+    return levels ? (int)digits_fit_31(p, n, levels) : -1;
+}
 
 // The plan make_plan picks (no tables): C, R, row3, row3m, rBB, rBC, col3, K; 0 or -1 if rejected.
 extern "C" int emu_plan(uint32_t p, uint32_t n, int lc, int row3, uint32_t *out) {

@@ -98,6 +98,9 @@ def test_emulated_16384_word_rows_match_oracle(row3):
 COL3_CASES = [  # (p, n, lc, K): 3-phase columns, R = 512, 1024, 2048, 4096
     (1000003, 65536, 7, 8), (1000003, 65536, 7, 16), (2000003, 131072, 7, 8), (2000003, 131072, 7, 16),
     (4000037, 262144, 7, 4), (4000037, 262144, 7, 8), (8000009, 524288, 7, 4),
+    # 8 bits per digit: carried digits of arbitrary inputs may exceed 2^31, so the weight
+    # multiply takes the field-encoded digit (no biased form)
+    (524309, 65536, 7, 16), (1048583, 131072, 7, 8),
 ]
 
 
@@ -310,3 +313,48 @@ def test_plan_fallback_is_the_default_plan():
                         assert used in (0, 1) and ok.value == 1, (n, lc, K, r3, c3)
                         if used == 0:
                             assert tuple(out) == default, (n, lc, K, r3, c3, tuple(out), default)
+
+
+def test_biased_weight_multiply_matches_python():
+    """gl_mul_biased(d + 2^31, w, -2^31 w mod P) = d w mod P for every 32-bit biased digit,
+    edge operands and random ones (the column kernels' weight multiply on carried digits)."""
+    import random
+    L = emu()
+    L.emu_gl_mul_biased.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64]
+    L.emu_gl_mul_biased.restype = ctypes.c_uint64
+    P = (1 << 64) - (1 << 32) + 1
+    rng = random.Random(31)
+    ws = [0, 1, 2, P - 1, P - 2, (1 << 32) - 1, 1 << 32, (1 << 63), P >> 1] + [rng.randrange(P) for _ in range(200)]
+    ds = [0, 1, (1 << 31) - 1, 1 << 31, (1 << 31) + 1, (1 << 32) - 1] + [rng.randrange(1 << 32) for _ in range(200)]
+    for w in ws:
+        wb = (-(1 << 31) * w) % P
+        for db in ds:
+            assert L.emu_gl_mul_biased(db, w, wb) == ((db - (1 << 31)) * w) % P, (db, w)
+
+
+def test_biased_digit_bound():
+    """digits_fit_31: after the plan's carry levels every digit of any input below 2^63 stays
+    below 2^31, checked by iterating the level bound in Python; true at the 2^22 headline
+    exponent, false where few digit bits leave large carries."""
+    L = emu()
+    L.emu_digits_fit_31.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+    L.emu_digits_fit_31.restype = ctypes.c_int
+    L.emu_carry_levels.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+    L.emu_carry_levels.restype = ctypes.c_uint32
+    seen = set()
+    for p, n in [(70000027, 1 << 22), (140000041, 1 << 23), (2000003, 1 << 17), (1000003, 1 << 16),
+                 (637534199, 1 << 25), (524309, 1 << 16), (655373, 1 << 16), (600011, 1 << 16),
+                 (1048583, 1 << 17)]:
+        m = L.emu_carry_levels(p, n)
+        if not m:
+            continue
+        L2 = n.bit_length() - 1
+        wmin, wmax = p >> L2, (p + n - 1) >> L2
+        V = 1 << 63
+        for _ in range(m):
+            V = (1 << (wmax - 1)) + (V >> wmin) + 1
+        want = int(V < (1 << 31))
+        assert L.emu_digits_fit_31(p, n) == want, (p, n)
+        seen.add(want)
+    assert L.emu_digits_fit_31(70000027, 1 << 22) == 1
+    assert seen == {0, 1}
