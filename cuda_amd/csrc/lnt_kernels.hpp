@@ -184,16 +184,27 @@ LNT_KFN void lnt_col_fwd_body(int tid, int bid, LView lds, const LntParams &P, b
 #pragma unroll
         for (int j = 0; j < K; j++) v[M + j] = gl_to_i64(lds[(u32)(r * KS + j)]);
         lnt_carry_run<M, K>(v, P.p, P.log_n, g0, P.levels, sub2);
+        // the digit as d + 2^31 for the biased weight multiply, else field-encoded
 #pragma unroll
-        for (int j = 0; j < K; j++) lds[(u32)(r * KS + j)] = gl_from_i64(v[M + j]);
+        for (int j = 0; j < K; j++)
+            lds[(u32)(r * KS + j)] = P.bias ? (u64)(u32)(v[M + j] + (int64_t(1) << 31)) : gl_from_i64(v[M + j]);
     }
     LNT_SYNC();
 
     // 2. phase 1: weight, B1-point DFT down the column, inter-block twiddle
+    if (P.bias) {
 #pragma unroll
-    for (int j = 0; j < B1; j++)
-        a[j] = gl_mul(lds[(u32)((t + B2 * j) * KS + cl)],
-                      PF ? wv[PF ? j : 0] : P.w[2 * lnt_wslot(ec, mC, P.n, (u32)(t + B2 * j))]);
+        for (int j = 0; j < B1; j++) {
+            const u32 sl = lnt_wslot(ec, mC, P.n, (u32)(t + B2 * j));
+            a[j] = gl_mul_biased((u32)lds[(u32)((t + B2 * j) * KS + cl)], PF ? wv[PF ? j : 0] : P.w[2 * sl],
+                                 P.w[2 * sl + 1]);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < B1; j++)
+            a[j] = gl_mul(lds[(u32)((t + B2 * j) * KS + cl)],
+                          PF ? wv[PF ? j : 0] : P.w[2 * lnt_wslot(ec, mC, P.n, (u32)(t + B2 * j))]);
+    }
     dft_fwd<B1>(a);
     if (B2 > 1) {
 #pragma unroll
