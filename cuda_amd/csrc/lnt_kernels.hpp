@@ -45,8 +45,9 @@ struct LntParams {
     GView<u64> x;            // n words: residue state (pre-carry values between iterations)
     GView<u64> y;            // n words: transform-domain work buffer
     GView<const u64> w, uw;  // IBDWT weights, unweights * n^-1: n words in digit order (one-workgroup
-                             // path); large path: row factors indexed by lnt_wslot, the weights
-                             // as pairs (w, -2^31 w) at 2 * slot, the unweights at slot
+                             // path: the weights as pairs (w, -2^31 w) at 2i, the unweights at i);
+                             // large path: row factors indexed by lnt_wslot, the weights as
+                             // pairs at 2 * slot, the unweights at slot
     GView<const u64> tw4, itw4;  // 4-step twiddles (large path), natural slots, times the column
                                  // weight factor (unweight factor * n^-1)
     GView<const u64> twr, itwr;  // row-line twiddles (C words)
@@ -781,7 +782,9 @@ LNT_KFN void lnt_small_body(int tid, LView lds, const LntParams &P, u32 iters, b
                 for (int j = 0; j < M; j++) halo[j] = v[PK + j];
                 lnt_carry_run<M, PK>(v, P.p, P.log_n, g0 + s * PK, P.levels, sub2);
 #pragma unroll
-                for (int j = 0; j < PK; j++) lds[g0 + s * PK + j] = gl_from_i64(v[M + j]);
+                for (int j = 0; j < PK; j++)
+                    lds[g0 + s * PK + j] =
+                        P.bias ? (u64)(u32)(v[M + j] + (int64_t(1) << 31)) : gl_from_i64(v[M + j]);
             }
             LNT_SYNC();
         }
@@ -789,7 +792,7 @@ LNT_KFN void lnt_small_body(int tid, LView lds, const LntParams &P, u32 iters, b
 #pragma unroll
         for (int j = 0; j < B1; j++) {
             const u32 i = t + B2 * j;
-            a[j] = gl_mul(lds[i], Tw[i]);
+            a[j] = P.bias ? gl_mul_biased((u32)lds[i], Tw[2 * i], Tw[2 * i + 1]) : gl_mul(lds[i], Tw[2 * i]);
         }
         dft_fwd<B1>(a);
         if (B2 > 1) {
@@ -884,7 +887,8 @@ LNT_KFN void lnt_small3_body(int tid, LView lds, const LntParams &P, u32 iters, 
             LNT_SYNC();  // every halo read before any chunk is overwritten
             lnt_carry_run<M, B>(v, P.p, P.log_n, g0, P.levels, it == 0 ? sub2_first : ll);
 #pragma unroll
-            for (int j = 0; j < B; j++) lds[g0 + j] = gl_from_i64(v[M + j]);
+            for (int j = 0; j < B; j++)
+                lds[g0 + j] = P.bias ? (u64)(u32)(v[M + j] + (int64_t(1) << 31)) : gl_from_i64(v[M + j]);
             LNT_SYNC();
         }
         u64 a[B];
@@ -892,7 +896,7 @@ LNT_KFN void lnt_small3_body(int tid, LView lds, const LntParams &P, u32 iters, 
 #pragma unroll
         for (int j = 0; j < B; j++) {
             const u32 i = (u32)(t + T * j);
-            a[j] = gl_mul(lds[i], Tw[i]);
+            a[j] = P.bias ? gl_mul_biased((u32)lds[i], Tw[2 * i], Tw[2 * i + 1]) : gl_mul(lds[i], Tw[2 * i]);
         }
         dft_fwd<B>(a);
 #pragma unroll

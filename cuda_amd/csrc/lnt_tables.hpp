@@ -12,17 +12,14 @@ namespace lnt {
 
 // ------------------------------------------------------------------ table kernels
 
+// one-workgroup path: digit weights as (w_i, -2^31 w_i) pairs (2n words), unweights (n words)
 static __global__ void lnt_k_weights(u64 Bw, u64 Binv, u64 ninv, u32 p, u32 n, u64 *w, u64 *uw) {
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (i == 0) {
-        w[0] = 1;
-        uw[0] = ninv;
-        return;
-    }
-    u32 r = (u32)((u64)p * i) & (n - 1);
-    w[i] = gl_pow(Bw, (u64)(n - r));
-    uw[i] = gl_mul(gl_pow(Binv, (u64)(n - r)), ninv);
+    const u64 wi = lnt_weight(Bw, p, n, (u32)i);
+    w[2 * i] = wi;
+    w[2 * i + 1] = gl_mul(wi, GL_P - (1ull << 31));
+    uw[i] = i ? gl_mul(gl_pow(Binv, (u64)(n - ((u32)((u64)p * i) & (n - 1)))), ninv) : ninv;
 }
 
 // large path: L = 2R row slots; weights as (w, -2^31 w) pairs (2L words), unweights (L words)
@@ -60,7 +57,7 @@ static GView<T> view(T *p, u64 n) {
 // Word counts of the tables of a plan (0 where the plan has no such table).
 struct TableSizes {
     u32 n, Lr, L2, R, A;  // residue, row line, second-level row line, column line, second-level column line
-    u32 W, Wu;            // weight / unweight table words: n each (one workgroup) or 4R / 2R
+    u32 W, Wu;            // weight / unweight table words: 2n / n (one workgroup) or 4R / 2R
     bool large;
 };
 static TableSizes table_sizes(const Plan &pl) {
@@ -71,7 +68,7 @@ static TableSizes table_sizes(const Plan &pl) {
     t.L2 = pl.row3m ? pl.rBB * pl.rBC : pl.row3 ? pl.row3 * pl.row3 : 0;
     t.R = pl.small ? 0 : pl.R;
     t.A = (!pl.small && pl.col3) ? pl.R / pl.col3 : 0;
-    t.W = pl.small ? pl.n : 4 * pl.R;
+    t.W = pl.small ? 2 * pl.n : 4 * pl.R;
     t.Wu = pl.small ? pl.n : 2 * pl.R;
     return t;
 }
@@ -164,7 +161,7 @@ static LntParams params_of(const Plan &pl, u32 p, u64 *x, const TablePtrs &t) {
     P.C = pl.C;
     P.R = pl.R;
     P.levels = pl.levels;
-    P.bias = (!pl.small && pl.bias) ? 1u : 0u;
+    P.bias = pl.bias ? 1u : 0u;
     return P;
 }
 

@@ -63,12 +63,14 @@ extern "C" void *emu_create3(uint32_t p, uint32_t n, int prp, uint32_t lc, uint3
     u64 Bw = gl_pow(7, (GL_P - 1) / 192 / n * 5);
     u64 Binv = gl_inv(Bw);
     u64 ninv = gl_inv(n);
-    if (pl.small) {
-        e->w.resize(n);
+    if (pl.small) {  // weights as (w, -2^31 w) pairs
+        e->w.resize(2 * n);
         e->uw.resize(n);
         for (uint32_t i = 0; i < n; i++) {
-            e->w[i] = lnt_weight(Bw, p, n, i);
-            e->uw[i] = gl_mul(gl_inv(e->w[i]), ninv);
+            const u64 wi = lnt_weight(Bw, p, n, i);
+            e->w[2 * i] = wi;
+            e->w[2 * i + 1] = gl_mul(wi, GL_P - (1ull << 31));
+            e->uw[i] = gl_mul(gl_inv(wi), ninv);
         }
     } else {  // row factors (weights as pairs); the column factors go into tw4 / itw4 below
         e->w.resize(4 * pl.R);
@@ -151,7 +153,7 @@ extern "C" void *emu_create3(uint32_t p, uint32_t n, int prp, uint32_t lc, uint3
     P.C = pl.C;
     P.R = pl.R;
     P.levels = pl.levels;
-    P.bias = (!pl.small && pl.bias) ? 1u : 0u;
+    P.bias = pl.bias ? 1u : 0u;
     std::vector<uint64_t> d(n, 0);
     d[0] = prp ? 3 : 4;
     to_device_digits(e->wid, d.data(), e->x.data());
