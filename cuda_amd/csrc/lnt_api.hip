@@ -39,10 +39,16 @@ struct lnt_ctx {
     u64 *twc2 = nullptr, *itwc2 = nullptr;
     u64 *twr2 = nullptr, *itwr2 = nullptr;
     u64 *f = nullptr;  // transform of the second operand of a product (allocated on first use)
-    // GRAPH_ITERS iterations of the large path on x (each owing its "-2" when LL), captured
-    // once and replayed: consecutive launches from a graph start sooner than from the stream
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
+    // GRAPH_ITERS iterations of the large path on one residue buffer (each owing its "-2"
+    // when LL), captured once per buffer and replayed: consecutive launches from a graph
+    // start sooner than from the stream.  Slot 0 serves the residue x, slot 1 the Gerbicz
+    // check's squarings of its copy of D.
+    struct IterGraph {
+        u64 *buf = nullptr;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    IterGraph graphs[2];
     LntParams P{};
     std::string err;
     std::string tune_report;  // "name = ms" per candidate of the last lnt_tune
@@ -165,12 +171,15 @@ static LntParams params_for(const lnt_ctx *c, u64 *x, u64 *y, const u64 *f) {
     return P;
 }
 
-// k squarings (and -2 for LL) of the residue in buffer x; large path: three launches each.
+static void drop_graph(lnt_ctx::IterGraph &ig) {
+    if (ig.exec) (void)hipGraphExecDestroy(ig.exec);
+    if (ig.graph) (void)hipGraphDestroy(ig.graph);
+    ig = lnt_ctx::IterGraph{};
+}
+
+// Drops every captured iteration graph (each names the buffers and kernels of its capture).
 static void drop_graph(lnt_ctx *c) {
-    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
-    c->graph_exec = nullptr;
-    c->graph = nullptr;
+    for (auto &ig : c->graphs) drop_graph(ig);
 }
 
 // Iterations per replay of the captured iteration graph.
@@ -180,25 +189,30 @@ static bool launch_iteration(const Plan &pl, const LntParams &P, bool sub2, hipS
     return lnt_launch_col_fwd(pl, P, sub2, s) && lnt_launch_row(pl, P, s) && lnt_launch_col_inv(pl, P, s);
 }
 
-// Capture GRAPH_ITERS iterations of the residue buffer c->x (carry with "-2" for LL).
-static int build_graph(lnt_ctx *c) {
-    drop_graph(c);
-    const LntParams P = params_for(c, c->x, c->y, nullptr);
+// Capture GRAPH_ITERS iterations of the residue buffer x into ig (carry with "-2" for LL).
+static int build_graph(lnt_ctx *c, lnt_ctx::IterGraph &ig, u64 *x) {
+    drop_graph(ig);
+    const LntParams P = params_for(c, x, c->y, nullptr);
     const bool ll = !c->prp;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     bool ok = true;
     for (uint32_t i = 0; i < GRAPH_ITERS && ok; i++) ok = launch_iteration(c->pl, P, ll, c->stream);
-    hipError_t ec = hipStreamEndCapture(c->stream, &c->graph);
+    hipError_t ec = hipStreamEndCapture(c->stream, &ig.graph);
     if (!ok || ec != hipSuccess) {
-        drop_graph(c);
+        drop_graph(ig);
         return fail(c, ok ? LNT_DEVICE_ERROR : LNT_UNSUPPORTED, "iteration graph capture failed");
     }
-    HIPCHK(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (hipGraphInstantiate(&ig.exec, ig.graph, nullptr, nullptr, 0) != hipSuccess) {
+        drop_graph(ig);
+        return fail(c, LNT_DEVICE_ERROR, "hipGraphInstantiate failed");
+    }
+    ig.buf = x;
     return LNT_OK;
 }
 
 // k squarings (and -2 for LL) of the residue in buffer x; large path: three launches each
-// (on the context's own residue, whole groups of GRAPH_ITERS run as one graph launch).
+// (on the context's own residue and on the Gerbicz check's copy, whole groups of
+// GRAPH_ITERS run as one graph launch).
 static int square_buf(lnt_ctx *c, u64 *x, bool &pending, uint32_t k) {
     const Plan &pl = c->pl;
     const bool ll = !c->prp;
@@ -221,13 +235,14 @@ static int square_buf(lnt_ctx *c, u64 *x, bool &pending, uint32_t k) {
             pending = true;
             k--;
         }
-        if (x == c->x && k >= GRAPH_ITERS) {
-            if (!c->graph_exec) {
-                int rc = build_graph(c);
+        lnt_ctx::IterGraph *ig = x == c->x ? &c->graphs[0] : x == c->g.E.x ? &c->graphs[1] : nullptr;
+        if (ig && k >= GRAPH_ITERS) {
+            if (!ig->exec || ig->buf != x) {
+                int rc = build_graph(c, *ig, x);
                 if (rc != LNT_OK) return rc;
             }
             for (; k >= GRAPH_ITERS; k -= GRAPH_ITERS)
-                HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch");
+                HIPCHK(c, hipGraphLaunch(ig->exec, c->stream), "hipGraphLaunch");
             pending = true;
         }
         for (uint32_t i = 0; i < k; i++) {
