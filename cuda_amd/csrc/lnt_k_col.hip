@@ -50,14 +50,24 @@ bool lnt_launch_col_fwd(const lnt::Plan &pl, const LntParams &P, bool sub2, hipS
     return false;
 }
 
+// lnt_k_col_ilp.hip (built with another machine scheduler)
+bool lnt_launch_col3_inv_16x8x8_k16(const lnt::Plan &pl, const LntParams &P, hipStream_t s);
+
+template <int BA, int BB, int BC, int K>
+static bool go_col3_inv(const lnt::Plan &pl, const LntParams &P, hipStream_t s) {
+    if constexpr (BA == 16 && BB == 8 && BC == 8 && K == 16) {
+        return lnt_launch_col3_inv_16x8x8_k16(pl, P, s);  // not instantiated in this file
+    } else {
+        hipLaunchKernelGGL((lnt_k_col3_inv<BA, BB, BC, K>), dim3(pl.C / K), dim3(K * BB * BC), 0, s, P);
+        return true;
+    }
+}
+
 bool lnt_launch_col_inv(const lnt::Plan &pl, const LntParams &P, hipStream_t s) {
     if (!lnt::plan_consistent(pl)) return false;  // a kernel of another shape would run out of bounds
     if (pl.col3) {
-#define C3I(ba, bb, bc, k)                                                                           \
-    if (pl.col3 == ba && pl.cBB == bb && pl.cBC == bc && pl.K == k) {                                \
-        hipLaunchKernelGGL((lnt_k_col3_inv<ba, bb, bc, k>), dim3(pl.C / k), dim3(k * bb * bc), 0, s, P); \
-        return true;                                                                                 \
-    }
+#define C3I(ba, bb, bc, k) \
+    if (pl.col3 == ba && pl.cBB == bb && pl.cBC == bc && pl.K == k) return go_col3_inv<ba, bb, bc, k>(pl, P, s);
         COL3_SHAPES(C3I)
 #undef C3I
         return false;
