@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <future>
 #include <map>
 #include <mutex>
 #include <string>
@@ -902,14 +903,24 @@ static int gz_alloc(lnt_ctx *c, lnt_ctx::Slot &s) {
 
 static int gz_check(lnt_ctx *c, bool &passed) {
     auto &g = c->g;
+    const u32 n = c->pl.n;
     int rc;
+    // D is final for this check: read it back first and put it in canonical digits on a
+    // host thread while the device runs the B squarings of E (the future's destructor
+    // waits for the thread on every return path, before xd and dd go away)
+    std::vector<uint64_t> xd(n), dd(n), de;
+    HIPCHK(c, hipMemcpyAsync(xd.data(), g.D.x, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpy (D2H)");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    const bool dsub2 = !c->prp && g.D.pending;
+    std::future<bool> dconv = std::async(std::launch::async, [c, &xd, &dd, dsub2] {
+        return to_canonical_digits(c->wid, xd.data(), dsub2, dd.data());
+    });
     if ((rc = copy_buf(c, g.E.x, g.Dprev.x))) return rc;
     g.E.pending = g.Dprev.pending;
     if ((rc = square_buf(c, g.E.x, g.E.pending, g.B))) return rc;
     if ((rc = mul_buf(c, g.E.x, g.E.pending, g.U0.x, g.U0.pending))) return rc;
-    std::vector<uint64_t> de, dd;
     if ((rc = canonical_of(c, g.E.x, g.E.pending, de))) return rc;
-    if ((rc = canonical_of(c, g.D.x, g.D.pending, dd))) return rc;
+    dconv.wait();
     passed = de == dd;
     if (passed) g.ck_res64 = res64(c->wid, dd.data());
     return LNT_OK;
