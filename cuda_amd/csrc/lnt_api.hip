@@ -11,6 +11,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <system_error>
 #include <vector>
 
 #include "../../include/lnt_api.h"
@@ -912,15 +913,20 @@ static int gz_check(lnt_ctx *c, bool &passed) {
     HIPCHK(c, hipMemcpyAsync(xd.data(), g.D.x, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpy (D2H)");
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     const bool dsub2 = !c->prp && g.D.pending;
-    std::future<bool> dconv = std::async(std::launch::async, [c, &xd, &dd, dsub2] {
-        return to_canonical_digits(c->wid, xd.data(), dsub2, dd.data());
-    });
+    std::future<bool> dconv;
+    try {
+        dconv = std::async(std::launch::async, [c, &xd, &dd, dsub2] {
+            return to_canonical_digits(c->wid, xd.data(), dsub2, dd.data());
+        });
+    } catch (const std::system_error &) {  // no thread available: convert here
+        to_canonical_digits(c->wid, xd.data(), dsub2, dd.data());
+    }
     if ((rc = copy_buf(c, g.E.x, g.Dprev.x))) return rc;
     g.E.pending = g.Dprev.pending;
     if ((rc = square_buf(c, g.E.x, g.E.pending, g.B))) return rc;
     if ((rc = mul_buf(c, g.E.x, g.E.pending, g.U0.x, g.U0.pending))) return rc;
     if ((rc = canonical_of(c, g.E.x, g.E.pending, de))) return rc;
-    dconv.wait();
+    if (dconv.valid()) dconv.wait();
     passed = de == dd;
     if (passed) g.ck_res64 = res64(c->wid, dd.data());
     return LNT_OK;
