@@ -213,7 +213,10 @@ inline std::string make_plan_fields(uint32_t p, uint64_t n, Plan &pl, PlanOpts o
     if (pl.levels == 0) return "NTT length too small for this exponent (convolution would overflow)";
     pl.bias = digits_fit_31(p, (uint32_t)n, pl.levels);
     pl.M = pl.levels <= 4 ? 4 : 8;
-    if (pl.log_n <= 12) {
+    // n <= 4096 runs in one workgroup except at n = 2048, where the one-workgroup kernel
+    // holds 64 words in each of 32 threads (121 us per iteration) and 16 rows of 128 words
+    // over three launches take 21.5 us; a row length may also be asked for at n = 4096
+    if (pl.log_n <= 10 || (pl.log_n == 12 && !force_lc)) {
         pl.small = true;
         pl.row = line_for(pl.log_n);
         pl.C = pl.n;
@@ -223,11 +226,15 @@ inline std::string make_plan_fields(uint32_t p, uint64_t n, Plan &pl, PlanOpts o
         return "";
     }
     uint32_t lc, lr;
-    if (!split_for(pl.log_n, lc, lr)) return "unsupported NTT length";
     if (force_lc) {
         lc = force_lc;
         lr = pl.log_n - lc;
         if (lc < 7 || lc > 14 || lr < 4 || lr > 12) return "unsupported row/column split";
+    } else if (pl.log_n == 11) {
+        lc = 7;
+        lr = 4;
+    } else if (!split_for(pl.log_n, lc, lr)) {
+        return "unsupported NTT length";
     }
     pl.C = 1u << lc;
     pl.R = 1u << lr;

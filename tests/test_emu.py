@@ -39,9 +39,11 @@ class Emu:
         return int(self.L.emu_res64(self.h))
 
 
-CASES = [  # (p, n, force_lc): small path 8..4096, large path 2^13..2^17 with several splits
+CASES = [  # (p, n, force_lc): one workgroup at 8..1024 and 4096, large path at 2048 and
+           # 2^13..2^17 with several splits
     (131, 8, 0), (521, 32, 0), (1279, 64, 0), (2203, 128, 0), (4423, 256, 0), (9689, 512, 0),
     (19937, 1024, 0), (44497, 2048, 0), (86243, 4096, 0),
+    (44497, 2048, 7), (86243, 4096, 7), (86243, 4096, 8),  # the large path at n = 2048 / 4096
     (86243, 8192, 0), (200003, 16384, 0), (400009, 32768, 0), (400009, 32768, 9),
     (1000003, 65536, 0), (1000003, 65536, 10), (2000003, 131072, 0),
     (1000003, 65536, 12), (2000003, 131072, 12),  # 4096-word rows over 16 / 32-row columns
