@@ -115,6 +115,32 @@ def test_every_tunable_plan_matches_oracle(p, n, monkeypatch):
     assert len(seen) >= 3
 
 
+@pytest.mark.parametrize("p,n,plans", [(44497, 2048, ["", "lc=7"]),
+                                       (86243, 4096, ["", "lc=7", "lc=8", "lc=8,K=8", "row3=0"])])
+def test_splits_at_2048_and_4096_match_oracle(p, n, plans, monkeypatch):
+    """n = 2048 (three launches by default) and 4096 (one workgroup by default, or the
+    three-kernel path when a row length is asked for): every plan against the oracle digit
+    for digit from a random residue (LL, three iterations)."""
+    rnd = random.Random(p + n)
+    o = OracleTest(p, n, False)
+    d = [rnd.getrandbits(wi) for wi in o.widths()]
+    o.set_digits(d)
+    o.iterate(3)
+    want = o.digits()
+    seen = set()
+    for plan in plans:
+        monkeypatch.setenv("LNT_PLAN", plan)
+        t = cuda_amd.LucasPRP(p, n)
+        desc = tuple(t.describe())
+        say("plan", n, plan or "default", ";".join(desc))
+        seen.add(desc)
+        t.set_digits(d)
+        t.iterate(3)
+        assert t.get_digits() == want, (n, plan, desc)
+        t.close()
+    assert len(seen) >= (1 if n == 2048 else 3)
+
+
 @pytest.mark.parametrize("p,n", [(318767093, 1 << 24), (600000001, 1 << 25), (1200000001, 1 << 26)])
 def test_longest_lengths_match_oracle(p, n):
     """n = 2^24, 2^25 and 2^26 (rows of 8192, 8192 and 16384 words, mixed-radix 3-phase row
