@@ -32,11 +32,21 @@ static bool go_small3_m(const LntParams &P, u32 iters, bool sub2_first, bool ll,
     return true;
 }
 
+// lnt_k_small_ilp.hip (n = 512, built with another machine scheduler)
+bool lnt_launch_small3_8(const lnt::Plan &pl, const LntParams &P, u32 iters, bool sub2_first, bool ll,
+                         hipStream_t s, int mode);
+bool lnt_launch_batch_small3_8(const lnt::Plan &pl, const LntParams *Ps, u32 count, u32 iters, bool ll,
+                               hipStream_t s);
+
 template <int B>
 static bool go_small3(const lnt::Plan &pl, const LntParams &P, u32 iters, bool sub2_first, bool ll, hipStream_t s,
                       int mode) {
-    return pl.M == 4 ? go_small3_m<B, 4>(P, iters, sub2_first, ll, s, mode)
-                     : go_small3_m<B, 8>(P, iters, sub2_first, ll, s, mode);
+    if constexpr (B == 8) {
+        return lnt_launch_small3_8(pl, P, iters, sub2_first, ll, s, mode);  // not instantiated here
+    } else {
+        return pl.M == 4 ? go_small3_m<B, 4>(P, iters, sub2_first, ll, s, mode)
+                         : go_small3_m<B, 8>(P, iters, sub2_first, ll, s, mode);
+    }
 }
 
 bool lnt_launch_small(const lnt::Plan &pl, const LntParams &P, u32 iters, bool sub2_first, bool ll, hipStream_t s,
@@ -63,12 +73,16 @@ static bool go_batch_small(const lnt::Plan &pl, const LntParams *Ps, u32 count, 
 
 template <int B>
 static bool go_batch_small3(const lnt::Plan &pl, const LntParams *Ps, u32 count, u32 iters, bool ll, hipStream_t s) {
-    dim3 grid(count), block(B * B);
-    if (pl.M == 4)
-        hipLaunchKernelGGL((lnt_kb_small3<B, 4>), grid, block, 0, s, Ps, iters, (int)ll);
-    else
-        hipLaunchKernelGGL((lnt_kb_small3<B, 8>), grid, block, 0, s, Ps, iters, (int)ll);
-    return true;
+    if constexpr (B == 8) {
+        return lnt_launch_batch_small3_8(pl, Ps, count, iters, ll, s);  // not instantiated here
+    } else {
+        dim3 grid(count), block(B * B);
+        if (pl.M == 4)
+            hipLaunchKernelGGL((lnt_kb_small3<B, 4>), grid, block, 0, s, Ps, iters, (int)ll);
+        else
+            hipLaunchKernelGGL((lnt_kb_small3<B, 8>), grid, block, 0, s, Ps, iters, (int)ll);
+        return true;
+    }
 }
 
 bool lnt_launch_batch_small(const lnt::Plan &pl, const LntParams *Ps, u32 count, u32 iters, bool ll, hipStream_t s) {
